@@ -1,0 +1,252 @@
+"""cuibm_amd — MI355X (gfx950) rebuild of cuIBM's per-time-step hot path.
+
+Python mirror of the reference's solver-class interface over the C ABI in
+``include/cuibm_amd.h`` (``libcuibm_amd.so``, built in-tree).  The compute runs in
+the hand-written HIP kernels of that library; there is no Python or CPU fallback:
+importing the package works anywhere, but creating a solver needs the built
+library and a GPU, and fails loudly otherwise.
+
+Reference interfaces mirrored (src/solvers/NavierStokes/NavierStokesSolver.h:145-186,
+src/cuIBM.cu:42-74)::
+
+    s = Solver("cases/cylinderRe40_luo")        # io::readInputs + factory
+    s.initialise()
+    while not s.finished():
+        s.step_time()
+        s.write_data()
+    s.shut_down()
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["Solver", "Csr", "CuibmError", "lib", "LIB_PATH", "CASES"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcuibm_amd.so")
+CASES = os.path.join(HERE, "cases")
+
+PRECONDITIONERS = {"NONE": 0, "DIAGONAL": 1, "SMOOTHED_AGGREGATION": 2}
+INT_FIELDS = {"ghostTagsUV", "hybridTagsUV", "hybridTagsUV2", "ghostTagsP", "hybridTagsP",
+              "LHS1_rowptr", "LHS1_col", "LHS2_rowptr", "LHS2_col", "tag_records_own", "tag_records_flags"}
+INFO_KEYS = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "fxx", "fxy", "fxu",
+             "startI", "startJ", "numCellsX", "numCellsY", "midX", "midY", "amg_levels", "nx", "ny",
+             "tol1", "tol2", "n_ghost_uv", "n_hybrid_uv", "n_ghost_p", "n_hybrid_p", "n_solid"]
+
+
+class CuibmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cuibm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+
+def lib():
+    """Load libcuibm_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C cuibm_amd/csrc")
+    L = C.CDLL(LIB_PATH)
+    L.cuibm_version.restype = C.c_int
+    L.cuibm_last_error.restype = C.c_char_p
+    L.cuibm_create.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_void_p)]
+    for f in ("cuibm_initialise", "cuibm_step_time", "cuibm_write_data", "cuibm_finished", "cuibm_shut_down"):
+        getattr(L, f).argtypes = [C.c_void_p]
+        getattr(L, f).restype = C.c_int
+    L.cuibm_destroy.argtypes = [C.c_void_p]
+    L.cuibm_destroy.restype = None
+    L.cuibm_name.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    L.cuibm_phase.argtypes = [C.c_void_p, C.c_char_p]
+    L.cuibm_case_array.argtypes = [C.c_void_p, C.c_char_p, _DP, C.c_long]
+    L.cuibm_case_array.restype = C.c_long
+    L.cuibm_case_scalars.argtypes = [C.c_void_p, _DP, C.c_int]
+    L.cuibm_field_size.argtypes = [C.c_void_p, C.c_char_p]
+    L.cuibm_field_size.restype = C.c_long
+    L.cuibm_get_field.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_long]
+    L.cuibm_get_field.restype = C.c_long
+    L.cuibm_set_field.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_long]
+    L.cuibm_set_field.restype = C.c_long
+    L.cuibm_info.argtypes = [C.c_void_p, _DP, C.c_int]
+    L.cuibm_bytes.argtypes = [C.c_void_p, C.c_int]
+    L.cuibm_bytes.restype = C.c_double
+    L.cuibm_time_steps.argtypes = [C.c_void_p, C.c_int, _DP]
+    L.cuibm_time_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _DP, _DP]
+    L.cuibm_csr_create.argtypes = [C.c_int, _IP, _IP, _DP, C.POINTER(C.c_void_p)]
+    L.cuibm_csr_destroy.argtypes = [C.c_void_p]
+    L.cuibm_csr_destroy.restype = None
+    L.cuibm_csr_spmv.argtypes = [C.c_void_p, _DP, _DP]
+    L.cuibm_bicgstab.argtypes = [C.c_void_p, _DP, _DP, C.c_int, C.c_int, C.c_double, C.c_int, _IP, _DP]
+    L.cuibm_amg_levels.argtypes = [C.c_void_p, _IP, _IP, C.c_int]
+    L.cuibm_amg_vcycle.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise CuibmError(rc, lib().cuibm_last_error().decode(errors="replace"))
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Solver:
+    """One cuIBM case on the GPU (NavierStokesSolver / luoIBM / fadlunModified)."""
+
+    def __init__(self, case_folder, *overrides, output=False, output_folder=None, diagnostics=True, **kw):
+        args = [str(a) for a in overrides]
+        for k, v in kw.items():
+            args += [f"-{k}", str(v)]
+        args += ["-output", "1" if output else "0", "-diagnostics", "1" if diagnostics else "0"]
+        if output_folder:
+            args += ["-outputFolder", str(output_folder)]
+        argv = (C.c_char_p * len(args))(*[a.encode() for a in args])
+        h = C.c_void_p()
+        _check(lib().cuibm_create(str(case_folder).encode(), len(args), argv, C.byref(h)))
+        self._h = h
+        self.case_folder = str(case_folder)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cuibm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # reference lifecycle
+    def initialise(self):
+        _check(lib().cuibm_initialise(self._h))
+        return self
+
+    def step_time(self, n=1):
+        for _ in range(n):
+            _check(lib().cuibm_step_time(self._h))
+
+    def write_data(self):
+        _check(lib().cuibm_write_data(self._h))
+
+    def finished(self):
+        return bool(_check(lib().cuibm_finished(self._h)))
+
+    def shut_down(self):
+        _check(lib().cuibm_shut_down(self._h))
+
+    def name(self):
+        buf = C.create_string_buffer(64)
+        _check(lib().cuibm_name(self._h, buf, 64))
+        return buf.value.decode()
+
+    def phase(self, name):
+        _check(lib().cuibm_phase(self._h, name.encode()))
+
+    # parsed case (host side, usable before initialise and without a GPU)
+    def case_array(self, name):
+        n = _check(lib().cuibm_case_array(self._h, name.encode(), None, 0))
+        a = np.zeros(n)
+        _check(lib().cuibm_case_array(self._h, name.encode(), _dp(a), n))
+        return a
+
+    CASE_KEYS = ["nx", "ny", "nu", "dt", "nt", "nsave", "SolverType", "scaleCV", "startStep", "tol1", "tol2",
+                 "maxit1", "maxit2", "precond1", "precond2", "amgPrecision", "uInitial", "vInitial", "uPerturb",
+                 "vPerturb", "bc_xm_u", "bc_xm_v", "bc_xp_u", "bc_xp_v", "bc_ym_u", "bc_ym_v", "bc_yp_u", "bc_yp_v",
+                 "mid_h", "nbodies"]
+
+    def case_scalars(self):
+        out = np.zeros(64)
+        n = _check(lib().cuibm_case_scalars(self._h, _dp(out), 64))
+        return dict(zip(self.CASE_KEYS, out[:n]))
+
+    # data access
+    def get(self, name):
+        n = _check(lib().cuibm_field_size(self._h, name.encode()))
+        a = np.zeros(n, dtype=np.int32 if name in INT_FIELDS else np.float64)
+        _check(lib().cuibm_get_field(self._h, name.encode(), a.ctypes.data_as(C.c_void_p), n))
+        return a
+
+    def set(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.int32 if name in INT_FIELDS else np.float64)
+        _check(lib().cuibm_set_field(self._h, name.encode(), a.ctypes.data_as(C.c_void_p), a.size))
+
+    def info(self):
+        out = np.zeros(32)
+        _check(lib().cuibm_info(self._h, _dp(out), 32))
+        return dict(zip(INFO_KEYS, out[: len(INFO_KEYS)]))
+
+    def bytes(self, which=0):
+        return lib().cuibm_bytes(self._h, which)
+
+    def time_steps(self, n):
+        ms = C.c_double(0)
+        _check(lib().cuibm_time_steps(self._h, n, C.byref(ms)))
+        return ms.value
+
+    def time_kernel(self, kernel, reps=20):
+        ms = C.c_double(0)
+        b = C.c_double(0)
+        _check(lib().cuibm_time_kernel(self._h, kernel.encode(), reps, C.byref(ms), C.byref(b)))
+        return ms.value, b.value
+
+
+class Csr:
+    """A square CSR matrix on the GPU with the BiCGSTAB / AMG entry points."""
+
+    def __init__(self, A):
+        A = A.tocsr()
+        A.sort_indices()
+        self.n = A.shape[0]
+        rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        ci = np.ascontiguousarray(A.indices, dtype=np.int32)
+        v = np.ascontiguousarray(A.data, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().cuibm_csr_create(self.n, rp.ctypes.data_as(_IP), ci.ctypes.data_as(_IP), _dp(v), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cuibm_csr_destroy(self._h)
+            self._h = None
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        _check(lib().cuibm_csr_spmv(self._h, _dp(x), _dp(y)))
+        return y
+
+    def bicgstab(self, b, x0=None, precond="NONE", rtol=1e-10, maxit=1000, amg_precision=64):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        it = C.c_int(0)
+        r = C.c_double(0)
+        rc = lib().cuibm_bicgstab(self._h, _dp(x), _dp(b), PRECONDITIONERS[precond], amg_precision, rtol, maxit,
+                                  C.byref(it), C.byref(r))
+        if rc < 0 and rc != -4:
+            _check(rc)
+        return x, it.value, r.value, rc == 0
+
+    def amg_levels(self):
+        s = np.zeros(64, dtype=np.int32)
+        z = np.zeros(64, dtype=np.int32)
+        L = _check(lib().cuibm_amg_levels(self._h, s.ctypes.data_as(_IP), z.ctypes.data_as(_IP), 64))
+        return s[:L], z[:L]
+
+    def vcycle(self, b, amg_precision=64):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        _check(lib().cuibm_amg_vcycle(self._h, amg_precision, _dp(b), _dp(x)))
+        return x

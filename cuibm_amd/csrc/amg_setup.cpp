@@ -1,0 +1,228 @@
+// Host set-up of the smoothed-aggregation hierarchy that replaces
+// cusp::precond::aggregation::smoothed_aggregation (src/preconditioner.h:101-104,
+// 142-145). CUSP 0.5.1's internal defaults are not available offline, so the method
+// is fixed here (Vanek, Mandel & Brezina 1996):
+//   strong connections: every nonzero off-diagonal entry of a row;
+//   aggregation: greedy three-phase standard aggregation in index order
+//     (1) a node whose neighbours are all free forms an aggregate with them,
+//     (2) a free node joins the aggregate of its first phase-1 neighbour,
+//     (3) a node still free forms an aggregate with its free neighbours;
+//   tentative prolongator: piecewise constant, each column scaled to unit norm;
+//   smoothed prolongator P = (I - 4/(3 rho) D^-1 A) T, rho = max_i sum_j |a_ij/a_ii|;
+//   restriction R = P^T, coarse operator A_c = R A P;
+//   coarsening stops at <= coarse_max unknowns, solved with a dense inverse.
+// Everything is sequential and in index order, so the hierarchy is deterministic.
+#include <algorithm>
+#include <cmath>
+
+#include "krylov.hpp"
+
+namespace cuibm {
+
+namespace {
+
+std::vector<double> diag_of(const HCsr &A) {
+    std::vector<double> d(A.n, 0.0);
+    for (int i = 0; i < A.n; i++)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++)
+            if (A.ci[k] == i) d[i] += A.v[k];
+    return d;
+}
+
+HCsr transpose(const HCsr &A) {
+    HCsr T;
+    T.n = A.ncols;
+    T.ncols = A.n;
+    T.rp.assign(T.n + 1, 0);
+    for (int c : A.ci) T.rp[c + 1]++;
+    for (int i = 0; i < T.n; i++) T.rp[i + 1] += T.rp[i];
+    T.ci.resize(A.ci.size());
+    T.v.resize(A.ci.size());
+    std::vector<int> next(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.n; i++)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) {
+            const int p = next[A.ci[k]]++;
+            T.ci[p] = i;
+            T.v[p] = A.v[k];
+        }
+    return T;
+}
+
+// C = A B with sorted columns; sums in the order (row of A) x (row of B)
+HCsr multiply(const HCsr &A, const HCsr &B) {
+    HCsr C;
+    C.n = A.n;
+    C.ncols = B.ncols;
+    C.rp.assign(A.n + 1, 0);
+    std::vector<double> acc(B.ncols, 0.0);
+    std::vector<int> seen(B.ncols, -1), cols;
+    for (int i = 0; i < A.n; i++) {
+        cols.clear();
+        for (int ka = A.rp[i]; ka < A.rp[i + 1]; ka++) {
+            const int k = A.ci[ka];
+            const double a = A.v[ka];
+            for (int kb = B.rp[k]; kb < B.rp[k + 1]; kb++) {
+                const int j = B.ci[kb];
+                if (seen[j] != i) {
+                    seen[j] = i;
+                    acc[j] = 0.0;
+                    cols.push_back(j);
+                }
+                acc[j] += a * B.v[kb];
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols) {
+            C.ci.push_back(j);
+            C.v.push_back(acc[j]);
+        }
+        C.rp[i + 1] = (int)C.ci.size();
+    }
+    return C;
+}
+
+int aggregate(const HCsr &A, std::vector<int> &agg) {
+    const int n = A.n;
+    agg.assign(n, -1);
+    auto strong = [&](int i, int k) { return A.ci[k] != i && A.v[k] != 0.0; };
+    int count = 0;
+    for (int i = 0; i < n; i++) {  // phase 1
+        if (agg[i] != -1) continue;
+        bool free_nbhd = true;
+        for (int k = A.rp[i]; k < A.rp[i + 1] && free_nbhd; k++)
+            if (strong(i, k) && agg[A.ci[k]] != -1) free_nbhd = false;
+        if (!free_nbhd) continue;
+        agg[i] = count;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++)
+            if (strong(i, k)) agg[A.ci[k]] = count;
+        count++;
+    }
+    const std::vector<int> first(agg);
+    for (int i = 0; i < n; i++) {  // phase 2
+        if (agg[i] != -1) continue;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++)
+            if (strong(i, k) && first[A.ci[k]] != -1) {
+                agg[i] = first[A.ci[k]];
+                break;
+            }
+    }
+    for (int i = 0; i < n; i++) {  // phase 3
+        if (agg[i] != -1) continue;
+        agg[i] = count;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++)
+            if (strong(i, k) && agg[A.ci[k]] == -1) agg[A.ci[k]] = count;
+        count++;
+    }
+    return count;
+}
+
+}  // namespace
+
+std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_levels) {
+    std::vector<HostAmgLevel> lv;
+    HCsr A = A0;
+    A.ncols = A.n;
+    while (A.n > coarse_max && (int)lv.size() < max_levels - 1) {
+        std::vector<int> agg;
+        const int nc = aggregate(A, agg);
+        if (nc >= A.n || nc == 0) break;
+        std::vector<int> size(nc, 0);
+        for (int a : agg) size[a]++;
+        HCsr T;
+        T.n = A.n;
+        T.ncols = nc;
+        T.rp.resize(A.n + 1);
+        T.ci = agg;
+        T.v.resize(A.n);
+        for (int i = 0; i <= A.n; i++) T.rp[i] = i;
+        for (int i = 0; i < A.n; i++) T.v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
+        const std::vector<double> d = diag_of(A);
+        double rho = 0;
+        for (int i = 0; i < A.n; i++) {
+            double s = 0;
+            for (int k = A.rp[i]; k < A.rp[i + 1]; k++) s += std::fabs(A.v[k] * (1.0 / d[i]));
+            rho = std::max(rho, s);
+        }
+        const double w = 4.0 / (3.0 * rho);
+        const HCsr AT = multiply(A, T);
+        HCsr P;
+        P.n = A.n;
+        P.ncols = nc;
+        P.rp.assign(A.n + 1, 0);
+        for (int i = 0; i < A.n; i++) {
+            const double di = 1.0 / d[i];
+            const int tc = T.ci[i];
+            const double tv = T.v[i];
+            bool placed = false;
+            for (int k = AT.rp[i]; k < AT.rp[i + 1]; k++) {
+                const int j = AT.ci[k];
+                if (!placed && tc < j) {
+                    P.ci.push_back(tc);
+                    P.v.push_back(tv);
+                    placed = true;
+                }
+                double val = -w * di * AT.v[k];
+                if (!placed && tc == j) {
+                    val = tv + val;
+                    placed = true;
+                }
+                P.ci.push_back(j);
+                P.v.push_back(val);
+            }
+            if (!placed) {
+                P.ci.push_back(tc);
+                P.v.push_back(tv);
+            }
+            P.rp[i + 1] = (int)P.ci.size();
+        }
+        HostAmgLevel L;
+        L.R = transpose(P);
+        const HCsr AP = multiply(A, P);
+        HCsr Ac = multiply(L.R, AP);
+        L.A = A;
+        L.P = P;
+        lv.push_back(std::move(L));
+        A = std::move(Ac);
+    }
+    HostAmgLevel last;
+    last.A = A;
+    lv.push_back(std::move(last));
+    return lv;
+}
+
+std::vector<double> dense_inverse(const HCsr &A) {
+    const int n = A.n;
+    std::vector<double> M((size_t)n * n, 0.0), X((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) M[(size_t)i * n + A.ci[k]] += A.v[k];
+        X[(size_t)i * n + i] = 1.0;
+    }
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs(M[(size_t)r * n + c]) > std::fabs(M[(size_t)piv * n + c])) piv = r;
+        if (M[(size_t)piv * n + c] == 0.0) throw Error("singular coarsest AMG operator", ERR_GENERIC);
+        if (piv != c)
+            for (int k = 0; k < n; k++) {
+                std::swap(M[(size_t)c * n + k], M[(size_t)piv * n + k]);
+                std::swap(X[(size_t)c * n + k], X[(size_t)piv * n + k]);
+            }
+        const double inv = 1.0 / M[(size_t)c * n + c];
+        for (int k = 0; k < n; k++) {
+            M[(size_t)c * n + k] *= inv;
+            X[(size_t)c * n + k] *= inv;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            const double f = M[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; k++) {
+                M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
+                X[(size_t)r * n + k] -= f * X[(size_t)c * n + k];
+            }
+        }
+    }
+    return X;
+}
+
+}  // namespace cuibm

@@ -1,0 +1,117 @@
+// HIP helpers: error checking, owning device arrays, launch geometry.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace cuibm {
+
+// Library error with a C-ABI return code (negative)
+struct Error : std::runtime_error {
+    int code;
+    explicit Error(const std::string &m, int c = -1) : std::runtime_error(m), code(c) {}
+};
+
+enum ErrorCode {
+    ERR_GENERIC = -1,
+    ERR_INPUT = -2,
+    ERR_HIP = -3,
+    ERR_NOT_CONVERGED = -4,
+    ERR_UNSUPPORTED = -5,
+    ERR_ARGUMENT = -6,
+};
+
+#define CUIBM_HIP(expr)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            throw ::cuibm::Error(std::string(#expr) + ": " + hipGetErrorString(e_) + " (" __FILE__ ":" + \
+                                     std::to_string(__LINE__) + ")",                                     \
+                                 ::cuibm::ERR_HIP);                                                      \
+    } while (0)
+
+#define CUIBM_LAUNCH_CHECK() CUIBM_HIP(hipGetLastError())
+
+constexpr int BS = 256;
+inline int nblk(long n, int bs = BS) { return (int)((n + bs - 1) / bs); }
+
+template <class T>
+class DArray {
+  public:
+    DArray() = default;
+    explicit DArray(size_t n) { alloc(n); }
+    DArray(const DArray &) = delete;
+    DArray &operator=(const DArray &) = delete;
+    DArray(DArray &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DArray &operator=(DArray &&o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    ~DArray() { release(); }
+    void alloc(size_t n) {
+        if (n == n_ && p_) return;
+        release();
+        n_ = n;
+        if (n) CUIBM_HIP(hipMalloc((void **)&p_, n * sizeof(T)));
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    void upload(const T *h, size_t n, hipStream_t s) {
+        alloc(n);
+        if (n) CUIBM_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    void download(T *h, size_t n, hipStream_t s) const {
+        if (n > n_) throw Error("download of " + std::to_string(n) + " > " + std::to_string(n_), ERR_ARGUMENT);
+        if (n) CUIBM_HIP(hipMemcpyAsync(h, p_, n * sizeof(T), hipMemcpyDeviceToHost, s));
+        CUIBM_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<T> to_host(hipStream_t s) const {
+        std::vector<T> h(n_);
+        download(h.data(), n_, s);
+        return h;
+    }
+    T *p() const { return p_; }
+    size_t n() const { return n_; }
+    bool empty() const { return n_ == 0; }
+
+  private:
+    T *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// device-side CSR matrix; values in V (double or float)
+template <class V>
+struct DCsr {
+    int n = 0, ncols = 0, nnz = 0;
+    DArray<int> rp, ci;
+    DArray<V> v;
+};
+
+// host-side CSR
+struct HCsr {
+    int n = 0, ncols = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+    int nnz() const { return (int)ci.size(); }
+};
+
+template <class V>
+void upload_csr(const HCsr &h, DCsr<V> &d, hipStream_t s) {
+    d.n = h.n;
+    d.ncols = h.ncols;
+    d.nnz = h.nnz();
+    d.rp.upload(h.rp, s);
+    d.ci.upload(h.ci, s);
+    std::vector<V> vv(h.v.begin(), h.v.end());
+    d.v.upload(vv, s);
+    CUIBM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace cuibm

@@ -1,0 +1,338 @@
+// Vector, reduction and sparse-product kernels for the Krylov and AMG solvers.
+//
+// Reductions are deterministic: a fixed grid of RED_GRID blocks strides over the
+// vector, each block reduces its threads in a fixed tree (wavefront shuffles, then
+// LDS across the 4 wavefronts of a 256-thread block) and writes one partial; a
+// single-block kernel then sums the partials in a fixed order. The result depends
+// only on n, never on scheduling.
+#include "krylov.hpp"
+#include "kernels.hpp"
+
+namespace cuibm {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// reduce `k` values per thread across a 256-thread block; thread 0 gets the sums
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
+    __shared__ double sh[K][BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) sh[k][wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = (sh[k][0] + sh[k][1]) + (sh[k][2] + sh[k][3]);
+}
+
+template <class T>
+__global__ void k_fill(T *x, T v, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+__global__ void k_copy(double *__restrict__ y, const double *__restrict__ x, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
+}
+__global__ void k_to_float(float *__restrict__ y, const double *__restrict__ x, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = (float)x[i];
+}
+__global__ void k_to_double(double *__restrict__ y, const float *__restrict__ x, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = (double)x[i];
+}
+
+static int grid_for(long n) {
+    const long b = (n + BS - 1) / BS;
+    return (int)(b < 65536 ? b : 65536);
+}
+
+void launch_fill(double *x, double v, long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(BS), 0, s, x, v, n);
+    CUIBM_LAUNCH_CHECK();
+}
+void launch_fill_f(float *x, float v, long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(BS), 0, s, x, v, n);
+    CUIBM_LAUNCH_CHECK();
+}
+void launch_copy(double *y, const double *x, long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(BS), 0, s, y, x, n);
+    CUIBM_LAUNCH_CHECK();
+}
+void launch_to_float(float *y, const double *x, long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(BS), 0, s, y, x, n);
+    CUIBM_LAUNCH_CHECK();
+}
+void launch_to_double(double *y, const float *x, long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_double, dim3(grid_for(n)), dim3(BS), 0, s, y, x, n);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// ---- sparse matrix-vector products --------------------------------------------------
+
+template <class V, class X>
+__device__ __forceinline__ X row_dot(const int *__restrict__ rp, const int *__restrict__ ci, const V *__restrict__ v,
+                                     const X *__restrict__ x, int i) {
+    X s = 0;
+    const int e = rp[i + 1];
+    for (int k = rp[i]; k < e; k++) s += (X)v[k] * x[ci[k]];
+    return s;
+}
+
+template <class V, class X>
+__global__ void __launch_bounds__(BS) k_spmv(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                             const V *__restrict__ v, const X *__restrict__ x, X *__restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = row_dot(rp, ci, v, x, i);
+}
+
+template <class V, class X>
+void launch_spmv(const DCsr<V> &A, const X *x, X *y, hipStream_t s) {
+    if (A.n <= 0) return;
+    hipLaunchKernelGGL((k_spmv<V, X>), dim3(grid_for(A.n)), dim3(BS), 0, s, A.n, A.rp.p(), A.ci.p(), A.v.p(), x, y);
+    CUIBM_LAUNCH_CHECK();
+}
+template void launch_spmv<double, double>(const DCsr<double> &, const double *, double *, hipStream_t);
+template void launch_spmv<float, float>(const DCsr<float> &, const float *, float *, hipStream_t);
+
+// y = A x with block partials of (y . a) and, when b2 is set, (y . y)
+__global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                 const double *__restrict__ v, const double *__restrict__ x,
+                                                 double *__restrict__ y, const double *__restrict__ a, int want_yy,
+                                                 double *__restrict__ part) {
+    double acc[2] = {0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double yi = row_dot(rp, ci, v, x, i);
+        y[i] = yi;
+        acc[0] += yi * a[i];
+        acc[1] += yi * yi;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        if (want_yy) part[RED_GRID + blockIdx.x] = acc[1];
+    }
+}
+
+void launch_spmv_dot(const DCsr<double> &A, const double *x, double *y, const double *a, bool want_yy, double *part,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_spmv_dot, dim3(RED_GRID), dim3(BS), 0, s, A.n, A.rp.p(), A.ci.p(), A.v.p(), x, y, a,
+                       (int)want_yy, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// r = b - A x; also p = r and rs = r when given; partials of (r.r) and (b.b)
+__global__ void __launch_bounds__(BS) k_bicg_init(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const double *__restrict__ v, const double *__restrict__ x,
+                                                  const double *__restrict__ b, double *__restrict__ r,
+                                                  double *__restrict__ p, double *__restrict__ rs,
+                                                  double *__restrict__ part) {
+    double acc[2] = {0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double bi = b[i];
+        const double ri = bi - row_dot(rp, ci, v, x, i);
+        r[i] = ri;
+        p[i] = ri;
+        rs[i] = ri;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        part[RED_GRID + blockIdx.x] = acc[1];
+    }
+}
+
+void launch_bicg_init(const DCsr<double> &A, const double *x, const double *b, double *r, double *p, double *rs,
+                      double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_init, dim3(RED_GRID), dim3(BS), 0, s, A.n, A.rp.p(), A.ci.p(), A.v.p(), x, b, r, p, rs,
+                       part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// s = r - alpha AMp with partials of (s.s)
+__global__ void __launch_bounds__(BS) k_bicg_s(int n, const double *__restrict__ r, const double *__restrict__ AMp,
+                                               double *__restrict__ s, const double *__restrict__ sc,
+                                               double *__restrict__ part) {
+    const double alpha = sc[SC_ALPHA];
+    double acc[1] = {0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double si = r[i] - alpha * AMp[i];
+        s[i] = si;
+        acc[0] += si * si;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const double *sc, double *part,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_s, dim3(RED_GRID), dim3(BS), 0, s, n, r, AMp, s_, sc, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// x += alpha Mp (early exit at the half step)
+__global__ void k_bicg_xhalf(int n, double *__restrict__ x, const double *__restrict__ Mp, const double *__restrict__ sc) {
+    const double alpha = sc[SC_ALPHA];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + alpha * Mp[i];
+}
+
+void launch_bicg_xhalf(int n, double *x, const double *Mp, const double *sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_xhalf, dim3(grid_for(n)), dim3(BS), 0, s, n, x, Mp, sc);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// x += alpha Mp + omega Ms; r = s - omega AMs; partials of (rs.r) and (r.r)
+__global__ void __launch_bounds__(BS) k_bicg_xr(int n, double *__restrict__ x, const double *__restrict__ Mp,
+                                                const double *__restrict__ Ms, double *__restrict__ r,
+                                                const double *__restrict__ s, const double *__restrict__ AMs,
+                                                const double *__restrict__ rs, const double *__restrict__ sc,
+                                                double *__restrict__ part) {
+    const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
+    double acc[2] = {0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] = x[i] + alpha * Mp[i] + omega * Ms[i];
+        const double ri = s[i] - omega * AMs[i];
+        r[i] = ri;
+        acc[0] += rs[i] * ri;
+        acc[1] += ri * ri;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        part[RED_GRID + blockIdx.x] = acc[1];
+    }
+}
+
+void launch_bicg_xr(int n, double *x, const double *Mp, const double *Ms, double *r, const double *s_,
+                    const double *AMs, const double *rs, const double *sc, double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_xr, dim3(RED_GRID), dim3(BS), 0, s, n, x, Mp, Ms, r, s_, AMs, rs, sc, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// p = r + beta p - beta omega AMp
+__global__ void k_bicg_p(int n, double *__restrict__ p, const double *__restrict__ r, const double *__restrict__ AMp,
+                         const double *__restrict__ sc) {
+    const double beta = sc[SC_BETA], omega = sc[SC_OMEGA];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = r[i] + beta * p[i] - beta * omega * AMp[i];
+}
+
+void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_p, dim3(grid_for(n)), dim3(BS), 0, s, n, p, r, AMp, sc);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// sum the partial groups and update the recurrence scalars (one block)
+__global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__ part, int op, double *__restrict__ sc) {
+    __shared__ double sh[2][16];
+    double v[2] = {0, 0};
+    const int ng = (op == FIN_ALPHA || op == FIN_SS) ? 1 : 2;
+    for (int g = 0; g < ng; g++) {
+        double t = 0;
+        for (int k = threadIdx.x; k < RED_GRID; k += blockDim.x) t += part[g * RED_GRID + k];
+        v[g] = t;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int g = 0; g < ng; g++) {
+        double t = wave_sum(v[g]);
+        if (lane == 0) sh[g][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double tot[2] = {0, 0};
+    for (int g = 0; g < ng; g++) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[g][w];
+        tot[g] = t;
+    }
+    switch (op) {
+        case FIN_INIT:  // (r.r, b.b)
+            sc[SC_RR] = tot[0];
+            sc[SC_RHO] = tot[0];
+            sc[SC_BB] = tot[1];
+            break;
+        case FIN_ALPHA:  // rs.AMp
+            sc[SC_ALPHA] = sc[SC_RHO] / tot[0];
+            break;
+        case FIN_SS:
+            sc[SC_SS] = tot[0];
+            break;
+        case FIN_OMEGA:  // (AMs.s, AMs.AMs)
+            sc[SC_OMEGA] = tot[0] / tot[1];
+            break;
+        case FIN_BETA: {  // (rs.r, r.r)
+            sc[SC_BETA] = (tot[0] / sc[SC_RHO]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
+            sc[SC_RHO] = tot[0];
+            sc[SC_RR] = tot[1];
+            break;
+        }
+        case FIN_DOT2:
+            sc[SC_D0] = tot[0];
+            sc[SC_D1] = tot[1];
+            break;
+    }
+}
+
+void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, part, op, sc);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// partial dot products (x.y) and (x.x)
+__global__ void __launch_bounds__(BS) k_dot2(long n, const double *__restrict__ x, const double *__restrict__ y,
+                                             double *__restrict__ part) {
+    double acc[2] = {0, 0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        acc[0] += x[i] * y[i];
+        acc[1] += x[i] * x[i];
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        part[RED_GRID + blockIdx.x] = acc[1];
+    }
+}
+
+void launch_dot2(long n, const double *x, const double *y, double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot2, dim3(RED_GRID), dim3(BS), 0, s, n, x, y, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// z = dinv * r (Jacobi)
+__global__ void k_jacobi_apply(int n, const double *__restrict__ dinv, const double *__restrict__ r,
+                               double *__restrict__ z) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = dinv[i] * r[i];
+}
+
+void launch_jacobi_apply(int n, const double *dinv, const double *r, double *z, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi_apply, dim3(grid_for(n)), dim3(BS), 0, s, n, dinv, r, z);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// dinv = 1 / (sum of the diagonal entries of each row)
+__global__ void k_diag_inv(int n, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+                           double *__restrict__ dinv) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double d = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++)
+            if (ci[k] == i) d += v[k];
+        dinv[i] = 1.0 / d;
+    }
+}
+
+void launch_diag_inv(const DCsr<double> &A, double *dinv, hipStream_t s) {
+    hipLaunchKernelGGL(k_diag_inv, dim3(grid_for(A.n)), dim3(BS), 0, s, A.n, A.rp.p(), A.ci.p(), A.v.p(), dinv);
+    CUIBM_LAUNCH_CHECK();
+}
+
+}  // namespace cuibm

@@ -1,0 +1,131 @@
+// Linear solvers: preconditioned BiCGSTAB with device-resident recurrence scalars,
+// Jacobi, and the smoothed-aggregation AMG V-cycle (FP64 or FP32 hierarchy).
+#pragma once
+#include <memory>
+#include <string>
+
+#include "dev.hpp"
+
+namespace cuibm {
+
+constexpr int RED_GRID = 2048;  // blocks of every reducing kernel
+
+// slots of the device scalar block
+enum { SC_BB = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_BETA, SC_SS, SC_RR, SC_D0, SC_D1, SC_COUNT = 16 };
+// finish-kernel operations
+enum { FIN_INIT = 0, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_BETA, FIN_DOT2 };
+
+// ---- k_blas.hip
+void launch_fill(double *x, double v, long n, hipStream_t s);
+void launch_fill_f(float *x, float v, long n, hipStream_t s);
+void launch_copy(double *y, const double *x, long n, hipStream_t s);
+void launch_to_float(float *y, const double *x, long n, hipStream_t s);
+void launch_to_double(double *y, const float *x, long n, hipStream_t s);
+template <class V, class X>
+void launch_spmv(const DCsr<V> &A, const X *x, X *y, hipStream_t s);
+void launch_spmv_dot(const DCsr<double> &A, const double *x, double *y, const double *a, bool want_yy, double *part,
+                     hipStream_t s);
+void launch_bicg_init(const DCsr<double> &A, const double *x, const double *b, double *r, double *p, double *rs,
+                      double *part, hipStream_t s);
+void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const double *sc, double *part, hipStream_t s);
+void launch_bicg_xhalf(int n, double *x, const double *Mp, const double *sc, hipStream_t s);
+void launch_bicg_xr(int n, double *x, const double *Mp, const double *Ms, double *r, const double *s_,
+                    const double *AMs, const double *rs, const double *sc, double *part, hipStream_t s);
+void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s);
+void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s);
+void launch_dot2(long n, const double *x, const double *y, double *part, hipStream_t s);
+void launch_jacobi_apply(int n, const double *dinv, const double *r, double *z, hipStream_t s);
+void launch_diag_inv(const DCsr<double> &A, double *dinv, hipStream_t s);
+
+// ---- preconditioners
+class Preconditioner {
+  public:
+    virtual ~Preconditioner() = default;
+    virtual void apply(const double *r, double *z, hipStream_t s) = 0;
+    virtual std::string name() const = 0;
+    // bytes moved by one apply, counted as distinct arrays touched per kernel
+    virtual double bytes_per_apply() const = 0;
+};
+
+class IdentityPrecond : public Preconditioner {
+  public:
+    explicit IdentityPrecond(int n) : n_(n) {}
+    void apply(const double *r, double *z, hipStream_t s) override { launch_copy(z, r, n_, s); }
+    std::string name() const override { return "NONE"; }
+    double bytes_per_apply() const override { return 16.0 * n_; }
+
+  private:
+    int n_;
+};
+
+class JacobiPrecond : public Preconditioner {
+  public:
+    JacobiPrecond(const DCsr<double> &A, hipStream_t s);
+    void apply(const double *r, double *z, hipStream_t s) override;
+    std::string name() const override { return "DIAGONAL"; }
+    double bytes_per_apply() const override { return 24.0 * n_; }
+
+  private:
+    int n_;
+    DArray<double> dinv_;
+};
+
+// smoothed-aggregation AMG (setup on the host, V-cycle on the device)
+struct AmgParams {
+    int precision = 64;   // 64 or 32
+    int coarse_max = 256;
+    int max_levels = 25;
+    double omega = 0.8;
+    int npre = 2, npost = 2;
+};
+
+struct AmgHierarchyInfo {
+    int levels = 0;
+    std::vector<int> n, nnz, nnzP;
+};
+
+class AmgPrecond : public Preconditioner {
+  public:
+    AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s);
+    ~AmgPrecond() override;
+    void apply(const double *r, double *z, hipStream_t s) override;
+    std::string name() const override { return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)" : "SMOOTHED_AGGREGATION"; }
+    double bytes_per_apply() const override;
+    AmgHierarchyInfo info() const;
+
+  private:
+    struct Impl;
+    std::unique_ptr<Impl> im_;
+    AmgParams prm_;
+};
+
+// host smoothed-aggregation set-up (amg_setup.cpp)
+struct HostAmgLevel {
+    HCsr A, P, R;
+};
+std::vector<HostAmgLevel> amg_build(const HCsr &A, int coarse_max, int max_levels);
+std::vector<double> dense_inverse(const HCsr &A);
+
+// ---- BiCGSTAB (krylov.hip)
+struct SolveStat {
+    int iterations = 0;
+    double residual = 0, tolerance = 0;
+    bool converged = false;
+};
+
+class Bicgstab {
+  public:
+    explicit Bicgstab(int n);
+    ~Bicgstab();
+    // cusp::krylov::bicgstab + cusp::monitor(b, maxit, rtol) semantics; x is the warm start
+    SolveStat solve(const DCsr<double> &A, double *x, const double *b, Preconditioner &M, double rtol, int maxit,
+                    hipStream_t s);
+    int n() const { return n_; }
+
+  private:
+    int n_;
+    DArray<double> r_, rs_, p_, Mp_, AMp_, s_, Ms_, AMs_, part_, sc_;
+    double *host_sc_ = nullptr;
+};
+
+}  // namespace cuibm

@@ -1,0 +1,922 @@
+// Host orchestration of one time step, mirroring (paths under src/solvers/NavierStokes/):
+//   NavierStokesSolver.cu:34-372         plain solver: initialise, stepTime, solves, output
+//   NavierStokes/*.cu                    generateRHS1/LHS1/RHS2/LHS2, velocityProjection
+//   luo_base.cu:28-285, luo_base/*.cu    IBM skeleton: _pre_step .. _post_step, tagging, force
+//   luoIBM.cu, luoIBM/*.cu               Luo external ghost-node method
+//   fadlunModified.cu, FadlunModified/*.cu  modified Fadlun direct forcing
+// The device state lives in this object; one HIP stream per solver. Non-convergence
+// is reported by throwing cuibm::Error(ERR_NOT_CONVERGED) instead of exit(-1).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iomanip>
+#include <sstream>
+
+#include "solver.hpp"
+
+namespace cuibm {
+
+// ============================================================================ plain solver
+
+NavierStokesSolver::NavierStokesSolver(ParameterDB *pDB, Domain *dInfo, const SolverOptions &o)
+    : db(pDB), dom(dInfo), opt(o) {}
+
+NavierStokesSolver::~NavierStokesSolver() {
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+GridDev NavierStokesSolver::grid() const { return GridDev{gx.p(), gdx.p(), gy.p(), gdy.p(), gxu.p(), gyv.p(), nx, ny}; }
+
+BcDev NavierStokesSolver::bcs() const {
+    return BcDev{bc[XMINUS].p(), bc[XPLUS].p(), bc[YMINUS].p(), bc[YPLUS].p()};
+}
+
+std::string NavierStokesSolver::outFolder() const { return opt.outputFolder.empty() ? db->caseFolder : opt.outputFolder; }
+
+void NavierStokesSolver::initialise() {
+    initialiseNoBody();
+    initialiseLHS();
+}
+
+// NavierStokesSolver.cu:45-139 with NavierStokes/cast.cu:9-95
+void NavierStokesSolver::initialiseNoBody() {
+    CUIBM_HIP(hipSetDevice(db->deviceNumber));
+    if (!stream_) CUIBM_HIP(hipStreamCreate(&stream_));
+    hipStream_t s = stream_;
+    nu = db->nu;
+    dt = db->dt;
+    nx = dom->nx;
+    ny = dom->ny;
+    numU = (nx - 1) * ny;
+    numV = (ny - 1) * nx;
+    numUV = numU + numV;
+    numP = nx * ny;
+    timeStep = db->startStep;
+    gx.upload(dom->x, s);
+    gdx.upload(dom->dx, s);
+    gy.upload(dom->y, s);
+    gdy.upload(dom->dy, s);
+    gxu.upload(dom->xu, s);
+    gyv.upload(dom->yv, s);
+    for (auto *a : {&u, &uhat, &uold, &N, &Nold, &L, &bc1, &rhs1}) {
+        a->alloc(numUV);
+        launch_fill(a->p(), 0.0, numUV, s);
+    }
+    for (auto *a : {&pressure, &rhs2}) {
+        a->alloc(numP);
+        launch_fill(a->p(), 0.0, numP, s);
+    }
+    // initialise.cu:26-63
+    std::vector<double> h(numUV);
+    const double xmin = dom->x[0], xmax = dom->x[nx - 1], ymin = dom->y[0], ymax = dom->y[ny - 1], pi = M_PI;
+    for (int idx = 0; idx < numU; idx++) {
+        const int i = idx % (nx - 1), j = idx / (nx - 1);
+        h[idx] = db->uInitial + db->uPerturb * std::cos(0.5 * pi * (2 * dom->xu[i] - xmax - xmin) / (xmax - xmin)) *
+                                    std::sin(pi * (2 * dom->yu[j] - ymax - ymin) / (ymax - ymin));
+    }
+    for (int idx = 0; idx < numV; idx++) {
+        const int i = idx % nx, j = idx / nx;
+        h[idx + numU] = db->vInitial + db->vPerturb * std::cos(0.5 * pi * (2 * dom->yv[j] - ymax - ymin) / (ymax - ymin)) *
+                                           std::sin(pi * (2 * dom->xv[i] - xmax - xmin) / (xmax - xmin));
+    }
+    u.upload(h, s);
+    launch_copy(uhat.p(), u.p(), numUV, s);
+    // boundary arrays, NavierStokesSolver.cu:108-128
+    std::vector<double> bx(2 * ny - 1), by(2 * nx - 1);
+    for (int loc : {XMINUS, XPLUS}) {
+        for (int i = 0; i < ny - 1; i++) {
+            bx[i] = db->bc[loc][0].value;
+            bx[i + ny] = db->bc[loc][1].value;
+        }
+        bx[ny - 1] = db->bc[loc][0].value;
+        bc[loc].upload(bx, s);
+    }
+    for (int loc : {YMINUS, YPLUS}) {
+        for (int i = 0; i < nx - 1; i++) {
+            by[i] = db->bc[loc][0].value;
+            by[i + nx - 1] = db->bc[loc][1].value;
+        }
+        by[2 * nx - 2] = db->bc[loc][1].value;
+        bc[loc].upload(by, s);
+    }
+    CUIBM_HIP(hipStreamSynchronize(s));
+    if (opt.output) {
+        makeDirectory(outFolder());
+        writeGrid(outFolder(), *dom);
+        iterationsFile.open(outFolder() + "/iterations");
+    }
+}
+
+void NavierStokesSolver::initialiseLHS() {
+    generateLHS1();
+    generateLHS2();
+    buildPreconditioners();
+}
+
+void NavierStokesSolver::generateLHS1() {
+    std::vector<int> rp;
+    lhs1_rowptr(nx, ny, rp);
+    LHS1.n = numUV;
+    LHS1.ncols = numUV;
+    LHS1.nnz = rp.back();
+    LHS1.rp.upload(rp, stream_);
+    LHS1.ci.alloc(LHS1.nnz);
+    LHS1.v.alloc(LHS1.nnz);
+    launch_lhs1(LHS1.rp.p(), LHS1.ci.p(), LHS1.v.p(), grid(), nu, dt, fadlunLhs(), stream_);
+}
+
+void NavierStokesSolver::generateLHS2() {
+    std::vector<int> rp;
+    lhs2_rowptr(nx, ny, rp);
+    LHS2.n = numP;
+    LHS2.ncols = numP;
+    LHS2.nnz = rp.back();
+    LHS2.rp.upload(rp, stream_);
+    LHS2.ci.alloc(LHS2.nnz);
+    LHS2.v.alloc(LHS2.nnz);
+    launch_lhs2(LHS2.rp.p(), LHS2.ci.p(), LHS2.v.p(), grid(), dt, centrePin(), fadlunLhs(), stream_);
+}
+
+static double csr_bytes(const DCsr<double> &A) { return A.nnz * 12.0 + 4.0 * (A.n + 1) + 16.0 * A.n; }
+
+static HCsr download_csr(const DCsr<double> &A, hipStream_t s) {
+    HCsr h;
+    h.n = A.n;
+    h.ncols = A.ncols;
+    h.rp = A.rp.to_host(s);
+    h.ci = A.ci.to_host(s);
+    h.v = A.v.to_host(s);
+    return h;
+}
+
+static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p, const DCsr<double> &A, hipStream_t s) {
+    switch (p.preconditioner) {
+        case NONE: return std::unique_ptr<Preconditioner>(new IdentityPrecond(A.n));
+        case DIAGONAL: return std::unique_ptr<Preconditioner>(new JacobiPrecond(A, s));
+        case SMOOTHED_AGGREGATION: {
+            AmgParams ap;
+            ap.precision = p.amgPrecision;
+            return std::unique_ptr<Preconditioner>(new AmgPrecond(download_csr(A, s), ap, s));
+        }
+        default: throw Error("preconditioner AINV is not supported", ERR_UNSUPPORTED);
+    }
+}
+
+// newPrecon.cu:17-35 / preconditioner.h:84-111
+void NavierStokesSolver::buildPreconditioners() {
+    PC1 = make_precond(db->velocitySolve, LHS1, stream_);
+    PC2 = make_precond(db->PoissonSolve, LHS2, stream_);
+    ks1.reset(new Bicgstab(numUV));
+    ks2.reset(new Bicgstab(numP));
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+}
+
+// NavierStokesSolver.cu:163-188
+void NavierStokesSolver::stepTime() {
+    generateRHS1();
+    solveIntermediateVelocity();
+    generateRHS2();
+    solvePoisson();
+    velocityProjection();
+    timeStep++;
+}
+
+// NavierStokes/intermediateVelocity.cu:14-38 (Nold = N is a buffer swap)
+void NavierStokesSolver::generateRHS1() {
+    std::swap(N, Nold);
+    launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
+                opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
+}
+
+void NavierStokesSolver::generateRHS2() { launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_); }
+
+void NavierStokesSolver::projectPlain() {
+    launch_project(u.p(), uhat.p(), uold.p(), pressure.p(), grid(), dt, nullptr, stream_);
+}
+
+void NavierStokesSolver::velocityProjection() { projectPlain(); }
+
+// NavierStokesSolver.cu:221-271
+void NavierStokesSolver::solveIntermediateVelocity() {
+    stat1 = ks1->solve(LHS1, uhat.p(), rhs1.p(), *PC1, db->velocitySolve.tolerance, db->velocitySolve.maxIterations,
+                       stream_);
+    iterationCount1 = stat1.iterations;
+    if (!stat1.converged) {
+        std::ostringstream m;
+        m << "ERROR: Solve for uhat failed at time step " << timeStep << "\nIterations   : " << iterationCount1
+          << "\nResidual norm: " << stat1.residual << "\nTolerance    : " << stat1.tolerance;
+        throw Error(m.str(), ERR_NOT_CONVERGED);
+    }
+}
+
+void NavierStokesSolver::solvePoisson() {
+    stat2 = ks2->solve(LHS2, pressure.p(), rhs2.p(), *PC2, db->PoissonSolve.tolerance, db->PoissonSolve.maxIterations,
+                       stream_);
+    iterationCount2 = stat2.iterations;
+    if (!stat2.converged) {
+        std::ostringstream m;
+        m << "ERROR: Solve for pressure failed at time step " << timeStep << "\nIterations   : " << iterationCount2
+          << "\nResidual norm: " << stat2.residual << "\nTolerance    : " << stat2.tolerance;
+        throw Error(m.str(), ERR_NOT_CONVERGED);
+    }
+}
+
+// NavierStokesSolver.cu:339-363
+void NavierStokesSolver::writeCommon() {
+    if (!opt.output) return;
+    if (timeStep % db->nsave == 0) ::cuibm::writeData(outFolder(), timeStep, uhat.to_host(stream_), pressure.to_host(stream_));
+    iterationsFile << timeStep << '\t' << iterationCount1 << '\t' << iterationCount2 << std::endl;
+}
+
+void NavierStokesSolver::writeData() { writeCommon(); }
+
+void NavierStokesSolver::shutDown() {
+    if (iterationsFile.is_open()) iterationsFile.close();
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+}
+
+// ---- field access
+DArray<double> *NavierStokesSolver::dfield(const std::string &n) {
+    if (n == "u") return &u;
+    if (n == "uhat") return &uhat;
+    if (n == "uold") return &uold;
+    if (n == "p" || n == "pressure") return &pressure;
+    if (n == "N") return &N;
+    if (n == "Nold") return &Nold;
+    if (n == "L") return &L;
+    if (n == "bc1") return &bc1;
+    if (n == "rhs1") return &rhs1;
+    if (n == "rhs2") return &rhs2;
+    if (n == "xm") return &bc[XMINUS];
+    if (n == "xp") return &bc[XPLUS];
+    if (n == "ym") return &bc[YMINUS];
+    if (n == "yp") return &bc[YPLUS];
+    if (n == "x") return &gx;
+    if (n == "dx") return &gdx;
+    if (n == "y") return &gy;
+    if (n == "dy") return &gdy;
+    if (n == "xu") return &gxu;
+    if (n == "yv") return &gyv;
+    if (n == "LHS1_val") return &LHS1.v;
+    if (n == "LHS2_val") return &LHS2.v;
+    return nullptr;
+}
+
+const DArray<double> *NavierStokesSolver::dfield(const std::string &n) const {
+    return const_cast<NavierStokesSolver *>(this)->dfield(n);
+}
+
+long NavierStokesSolver::fieldCount(const std::string &n) const {
+    if (auto *f = dfield(n)) return (long)f->n();
+    if (n == "LHS1_rowptr") return (long)LHS1.rp.n();
+    if (n == "LHS1_col") return (long)LHS1.ci.n();
+    if (n == "LHS2_rowptr") return (long)LHS2.rp.n();
+    if (n == "LHS2_col") return (long)LHS2.ci.n();
+    if (auto *f = const_cast<NavierStokesSolver *>(this)->ifield(n)) return (long)f->n();
+    return -1;
+}
+
+long NavierStokesSolver::getField(const std::string &n, void *out, long cap) {
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    if (auto *f = dfield(n)) {
+        const long c = std::min<long>(cap, (long)f->n());
+        f->download((double *)out, c, stream_);
+        return (long)f->n();
+    }
+    const DArray<int> *fi = nullptr;
+    if (n == "LHS1_rowptr") fi = &LHS1.rp;
+    else if (n == "LHS1_col") fi = &LHS1.ci;
+    else if (n == "LHS2_rowptr") fi = &LHS2.rp;
+    else if (n == "LHS2_col") fi = &LHS2.ci;
+    else fi = ifield(n);
+    if (fi) {
+        const long c = std::min<long>(cap, (long)fi->n());
+        fi->download((int *)out, c, stream_);
+        return (long)fi->n();
+    }
+    return -1;
+}
+
+long NavierStokesSolver::setField(const std::string &n, const void *in, long cnt) {
+    if (auto *f = dfield(n)) {
+        if (cnt != (long)f->n()) throw Error("setField " + n + ": size mismatch", ERR_ARGUMENT);
+        CUIBM_HIP(hipMemcpyAsync(f->p(), in, cnt * sizeof(double), hipMemcpyHostToDevice, stream_));
+        CUIBM_HIP(hipStreamSynchronize(stream_));
+        return cnt;
+    }
+    if (auto *fi = ifield(n)) {
+        if (cnt != (long)fi->n()) throw Error("setField " + n + ": size mismatch", ERR_ARGUMENT);
+        CUIBM_HIP(hipMemcpyAsync(fi->p(), in, cnt * sizeof(int), hipMemcpyHostToDevice, stream_));
+        CUIBM_HIP(hipStreamSynchronize(stream_));
+        return cnt;
+    }
+    return -1;
+}
+
+void NavierStokesSolver::info(double *out, int n) const {
+    std::vector<double> v(32, 0.0);
+    v[0] = timeStep;
+    v[1] = iterationCount1;
+    v[2] = iterationCount2;
+    v[3] = stat1.residual;
+    v[4] = stat2.residual;
+    const AmgHierarchyInfo ai = amgInfo();
+    v[16] = ai.levels;
+    v[17] = nx;
+    v[18] = ny;
+    v[19] = stat1.tolerance;
+    v[20] = stat2.tolerance;
+    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 32));
+}
+
+int NavierStokesSolver::phase(const std::string &n) {
+    if (n == "generateRHS1") generateRHS1();
+    else if (n == "solveIntermediateVelocity" || n == "solve1") solveIntermediateVelocity();
+    else if (n == "generateRHS2") generateRHS2();
+    else if (n == "solvePoisson" || n == "solve2") solvePoisson();
+    else if (n == "velocityProjection" || n == "project") velocityProjection();
+    else if (n == "generateLHS1") generateLHS1();
+    else if (n == "generateLHS2") generateLHS2();
+    else if (n == "rhs1_only") {  // the fused rhs1 sweep on the current N/Nold, without the swap
+        launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), L.p(), bc1.p(), grid(), bcs(), nu, dt, fadlunRhs(), stream_);
+    } else return -1;
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+std::string NavierStokesSolver::preconditionerName(int which) const {
+    const Preconditioner *p = which == 1 ? PC1.get() : PC2.get();
+    return p ? p->name() : "";
+}
+
+AmgHierarchyInfo NavierStokesSolver::amgInfo() const {
+    if (auto *a = dynamic_cast<const AmgPrecond *>(PC2.get())) return a->info();
+    return AmgHierarchyInfo();
+}
+
+
+// algorithmic bytes of one BiCGSTAB solve as run (distinct arrays per kernel)
+static double krylov_bytes(const DCsr<double> &A, const Preconditioner &M, const SolveStat &st) {
+    const double n = A.n, spmv = csr_bytes(A);
+    double b = spmv + 4 * 8 * n;  // init: r = b - Ax, writes r, p, rs
+    const double full = 2 * (M.bytes_per_apply() + spmv + 8 * n)  // two preconditioner + spmv-dot passes
+                        + 3 * 8 * n                                // s = r - alpha AMp
+                        + 8 * 8 * n                                // x, r update with dots
+                        + 4 * 8 * n;                               // p update
+    b += st.iterations * full;
+    if (st.converged && st.iterations >= 0) b += M.bytes_per_apply() + spmv + 8 * n + 3 * 8 * n + 3 * 8 * n;  // half step
+    return b;
+}
+
+double NavierStokesSolver::poissonBytes() const { return PC2 ? krylov_bytes(LHS2, *PC2, stat2) : 0.0; }
+double NavierStokesSolver::velocityBytes() const { return PC1 ? krylov_bytes(LHS1, *PC1, stat1) : 0.0; }
+double NavierStokesSolver::stepBytes() const {
+    // rhs1 sweep (u, Nold in; N, rhs1 out), rhs2 (uhat in, rhs2 out), projection (uhat, p, u in; u, uold out)
+    const double stencils = 32.0 * numUV + 8.0 * (numUV + numP) + 8.0 * (3 * numUV + numP);
+    return stencils + poissonBytes() + velocityBytes();
+}
+
+int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, double *bytes) {
+    hipStream_t s = stream_;
+    DArray<double> t1(std::max(numUV, numP)), t2(std::max(numUV, numP));
+    double b = 0;
+    auto once = [&]() {
+        if (k == "spmv_lhs2") launch_spmv<double, double>(LHS2, pressure.p(), t1.p(), s);
+        else if (k == "spmv_lhs1") launch_spmv<double, double>(LHS1, uhat.p(), t1.p(), s);
+        else if (k == "rhs1")
+            launch_rhs1(u.p(), Nold.p(), t1.p(), t2.p(), nullptr, nullptr, grid(), bcs(), nu, dt, fadlunRhs(), s);
+        else if (k == "vcycle") PC2->apply(rhs2.p(), t1.p(), s);
+        else throw Error("unknown kernel '" + k + "'", ERR_ARGUMENT);
+    };
+    if (k == "spmv_lhs2") b = csr_bytes(LHS2);
+    else if (k == "spmv_lhs1") b = csr_bytes(LHS1);
+    else if (k == "rhs1") b = 32.0 * numUV + 8.0 * (2 * nx + 2 * ny) + 4 * 8.0 * (2 * ny + 2 * nx);
+    else if (k == "vcycle") b = PC2 ? PC2->bytes_per_apply() : 0;
+    once();  // warm-up
+    hipEvent_t a, e;
+    CUIBM_HIP(hipEventCreate(&a));
+    CUIBM_HIP(hipEventCreate(&e));
+    CUIBM_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < reps; r++) once();
+    CUIBM_HIP(hipEventRecord(e, s));
+    CUIBM_HIP(hipEventSynchronize(e));
+    float t = 0;
+    CUIBM_HIP(hipEventElapsedTime(&t, a, e));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(e);
+    if (ms) *ms = t / std::max(reps, 1);
+    if (bytes) *bytes = b;
+    return 0;
+}
+
+// ============================================================================ IBM base
+
+luo_base::luo_base(ParameterDB *pDB, Domain *dInfo, const SolverOptions &o) : NavierStokesSolver(pDB, dInfo, o) {}
+
+// src/bodies.cu:257-326 on the host (the reference runs it as a one-thread kernel)
+void luo_base::boundingBox() {
+    const double scale = db->scaleCV;
+    double xmax = *std::max_element(bX.begin(), bX.end()), xmin = *std::min_element(bX.begin(), bX.end());
+    double ymax = *std::max_element(bY.begin(), bY.end()), ymin = *std::min_element(bY.begin(), bY.end());
+    const double dxb = xmax - xmin, dyb = ymax - ymin;
+    xmax += 0.5 * dxb * (scale - 1.0);
+    xmin -= 0.5 * dxb * (scale - 1.0);
+    ymax += 0.5 * dyb * (scale - 1.0);
+    ymin -= 0.5 * dyb * (scale - 1.0);
+    int i = 0, j = 0;
+    while (dom->x[i + 1] < xmin) i++;
+    while (dom->y[j + 1] < ymin) j++;
+    startI = i;
+    startJ = j;
+    while (dom->x[i] < xmax) i++;
+    while (dom->y[j] < ymax) j++;
+    numCellsX = i - startI;
+    numCellsY = j - startJ;
+}
+
+// luo_base.cu:28-107 with src/bodies.cu:20-185
+void luo_base::initialise() {
+    initialiseNoBody();
+    hipStream_t s = stream_;
+    if (db->bodies.empty()) throw Error(std::string(solverTypeName(db->solverType)) + " needs a body (bodies.yaml)", ERR_INPUT);
+    if (db->VIV != 0) throw Error("VIV body motion is not supported", ERR_UNSUPPORTED);
+    bX.clear();
+    bY.clear();
+    for (auto &b : db->bodies) {
+        bX.insert(bX.end(), b.X.begin(), b.X.end());
+        bY.insert(bY.end(), b.Y.begin(), b.Y.end());
+    }
+    const int n = (int)bX.size();
+    boundingBox();  // from the points before re-centring (bodies.cu:149-154)
+    double mx = 0, my = 0;
+    for (int i = 0; i < n; i++) { mx += bX[i]; my += bY[i]; }
+    mx /= n;
+    my /= n;
+    const BodyDesc &b0 = db->bodies[0];
+    const double t = dt * timeStep;
+    const double xnew = b0.xCoefficient * std::sin(2 * M_PI * b0.xfrequency * t + b0.xPhase);
+    const double unew = b0.uCoefficient * std::cos(2 * M_PI * b0.xfrequency * t + b0.uPhase);
+    const double uoldd = b0.uCoefficient * std::cos(2 * M_PI * b0.xfrequency * (t - dt) + b0.uPhase);
+    if (b0.xCoefficient != 0 || b0.uCoefficient != 0)
+        throw Error("prescribed body oscillation is not supported yet", ERR_UNSUPPORTED);
+    midX = xnew;
+    midY = my;
+    for (int i = 0; i < n; i++) bX[i] = midX + (bX[i] - mx);
+    bUB.assign(n, unew);
+    bVB.assign(n, 0.0);
+    uBk = uoldd;
+    vBk = 0;
+    dbx.upload(bX, s);
+    dby.upload(bY, s);
+    dbu.upload(bUB, s);
+    dbv.upload(bVB, s);
+    for (auto *a : {&ghostTagsUV, &hybridTagsUV, &hybridTagsUV2}) a->alloc(numUV);
+    for (auto *a : {&ghostTagsP, &hybridTagsP}) a->alloc(numP);
+    for (auto *a : {&bi_x, &bi_y, &ip_x, &ip_y, &dist_int, &dist_between, &uvb, &ustar}) {
+        a->alloc(numUV);
+        launch_fill(a->p(), 0.0, numUV, s);
+    }
+    for (auto *a : {&bip_x, &bip_y, &ipp_x, &ipp_y, &dist_u_body, &dist_v_body, &pstar}) {
+        a->alloc(numP);
+        launch_fill(a->p(), 0.0, numP, s);
+    }
+    forcePart.alloc(6 * 64);
+    forceOut.alloc(8);
+    launch_fill(forceOut.p(), 0.0, 8, s);
+    tagPoints();
+    if (opt.output) {
+        forceFile.open(outFolder() + "/forces");
+        midPositionFile.open(outFolder() + "/midPosition");
+    }
+}
+
+// luo_base/tagPoints.cu:17-73
+void luo_base::tagPoints() {
+    hipStream_t s = stream_;
+    // fills of luo_base/tagPoints.cu:26-40 (fill kernels, no memset)
+    auto ifill = [&](DArray<int> &a, int v) {
+        std::vector<int> h(a.n(), v);
+        a.upload(h, s);
+    };
+    ifill(ghostTagsUV, -1);
+    ifill(ghostTagsP, -1);
+    ifill(hybridTagsUV, -1);
+    ifill(hybridTagsP, -1);
+    ifill(hybridTagsUV2, -1);
+    launch_fill(dist_int.p(), 1.0, numUV, s);
+    launch_fill(dist_between.p(), 1.0, numUV, s);
+    for (auto *a : {&bi_x, &bi_y, &ip_x, &ip_y}) launch_fill(a->p(), 0.0, numUV, s);
+    BoxDev box;
+    box.i_start = startI;
+    box.j_start = startJ;
+    box.w = numCellsX;
+    box.h = numCellsY;
+    box.nthr = ((int)((numCellsX * (double)numCellsY - 0.5) / 256) + 1) * 256;
+    rec.alloc(box.nthr);
+    BodyDev B{dbx.p(), dby.p(), dbu.p(), dbv.p(), (int)bX.size(), midX, midY};
+    TagOut o{ghostTagsUV.p(), hybridTagsUV.p(), hybridTagsUV2.p(), ghostTagsP.p(), hybridTagsP.p(),
+             bi_x.p(), bi_y.p(), ip_x.p(), ip_y.p(), bip_x.p(), bip_y.p(), ipp_x.p(), ipp_y.p(),
+             dist_int.p(), dist_between.p(), uvb.p(), dist_u_body.p(), dist_v_body.p()};
+    const CompDev cu{gxu.p(), gy.p(), nx - 1, ny, nx - 1, 0};
+    const CompDev cv{gx.p(), gyv.p(), nx, ny - 1, nx, numU};
+    launch_tag_velocity(B, cu, box, false, rec, o, nx, s);
+    launch_tag_velocity_resolve(cu, box, B, false, rec, o, nx, ny, s);
+    launch_tag_velocity(B, cv, box, true, rec, o, nx, s);
+    launch_tag_velocity_resolve(cv, box, B, true, rec, o, nx, ny, s);
+    launch_tag_pressure(B, grid(), box, rec, s);
+    launch_tag_pressure_resolve(grid(), box, B, rec, o, s);
+    launch_zero_rows(ghostTagsP.p(), box, nx, nx, 0, ny, numP, s);
+    launch_zero_rows(ghostTagsUV.p(), box, nx - 1, nx - 1, 0, ny, numU, s);
+    launch_zero_rows(ghostTagsUV.p(), box, nx, nx - 1, numU, ny - 1, numUV, s);
+    CUIBM_HIP(hipStreamSynchronize(s));
+    buildStencils();
+    tagged = true;
+}
+
+// compact node lists and interpolation weights for the current tags
+void luo_base::buildStencils() {
+    hipStream_t s = stream_;
+    const std::vector<int> gUV = ghostTagsUV.to_host(s), hUV = hybridTagsUV.to_host(s);
+    const std::vector<int> gP = ghostTagsP.to_host(s), hP = hybridTagsP.to_host(s);
+    std::vector<int> lg, lh, ls, lgp, lhp;
+    for (int i = 0; i < numUV; i++) {
+        if (gUV[i] > 0) lg.push_back(i);
+        if (hUV[i] > 0) lh.push_back(i);
+    }
+    // setInsideVelocity's index map (luo_base/kernels/intermediateVelocity.cu:18-31):
+    // u everywhere, v only for I < nx-1 and inside the array
+    for (int i = 0; i < numU; i++) {
+        const int I = i % (nx - 1), J = i / (nx - 1);
+        const int iv = J * nx + I + numU;
+        if (gUV[i] == 0) ls.push_back(i);
+        if (iv < numUV && gUV[iv] == 0) ls.push_back(iv);
+    }
+    for (int i = 0; i < numP; i++) {
+        if (gP[i] > 0) lgp.push_back(i);
+        if (hP[i] > 0) lhp.push_back(i);
+    }
+    auto setup = [&](Stencils &st, const std::vector<int> &list, int nextra) {
+        st.n = (int)list.size();
+        const size_t m = std::max<size_t>(list.size(), 1);
+        st.node.upload(list, s);
+        if (list.empty()) st.node.alloc(1);
+        st.idx.alloc(4 * m);
+        st.coef.alloc(4 * m);
+        st.nrm.alloc(8 * m);
+        st.extra.alloc(nextra * m);
+        st.tmp.alloc(m);
+    };
+    setup(gnV, lg, 1);
+    setup(hnV, lh, 1);
+    setup(gnP, lgp, 3);
+    setup(hnP, lhp, 1);
+    solid.upload(ls, s);
+    if (ls.empty()) solid.alloc(1);
+    nsolid = (int)ls.size();
+    const IbmGeom m{ghostTagsUV.p(), hybridTagsUV.p(), ghostTagsP.p(), hybridTagsP.p(), bi_x.p(), bi_y.p(),
+                    ip_x.p(), ip_y.p(), bip_x.p(), bip_y.p(), ipp_x.p(), ipp_y.p()};
+    launch_gn_vel_coef(grid(), m, gnV, s);
+    launch_hn_vel_coef(grid(), m, hnV, s);
+    launch_hn_p_coef(grid(), m, hnP, s);
+    launch_gn_p_coef(grid(), m, gnP, s);
+    CUIBM_HIP(hipStreamSynchronize(s));
+}
+
+// luo_base.cu:208-233 (VIV strong coupling is not supported)
+void luo_base::stepTime() {
+    _pre_step();
+    _intermediate_velocity();
+    _pressure();
+    _project_velocity();
+    _update_body();
+    _post_step();
+}
+
+// luo_base.cu:235-248: pressure_old = p and uold = u are not read before they are
+// rewritten in this solver, Nold = N is a buffer swap
+void luo_base::_pre_step() {
+    std::swap(N, Nold);
+    uB0 = bUB.empty() ? 0 : bUB[0];
+    vB0 = bVB.empty() ? 0 : bVB[0];
+    uBk = uB0;
+    vBk = vB0;
+}
+
+// luo_base/projectVelocity.cu:11-53
+void luo_base::_project_velocity() {
+    projectPlain();
+    launch_gn_vel_apply(u.p(), gnV, numU, bUB[0], bVB[0], stream_);
+    launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_);
+}
+
+// luo_base.cu:255-264
+void luo_base::_update_body() {
+    if (timeStep >= db->startStep) {
+        moveBody();
+        updateSolver();
+    }
+}
+
+void luo_base::_post_step() { timeStep++; }
+
+// luo_base.cu:154-174: force, then the prescribed motion (a static body here)
+void luo_base::moveBody() { calculateForce(); }
+
+// luo_base.cu:137-147: for a static body the re-tag reproduces the same tags, so it
+// is only run when the body has moved
+void luo_base::updateSolver() {}
+
+void luo_base::calculateForce() {
+    launch_force(u.p(), uold.p(), pressure.p(), ghostTagsUV.p(), grid(), nu, dt, startI, startJ, numCellsX, numCellsY,
+                 forcePart.p(), forceOut.p(), stream_);
+}
+
+void luo_base::readForces() {
+    double f[8];
+    forceOut.download(f, 6, stream_);
+    fxx = f[0];
+    fxy = f[1];
+    fxu = f[2];
+    forceX = fxx + fxy + fxu;
+    forceY = f[3] + f[4] + f[5];
+}
+
+void luo_base::updateRobinBoundary() { launch_robin(u.p(), bc[XPLUS].p(), grid(), dt, stream_); }
+
+// luo_base.cu:119-128
+void luo_base::writeData() {
+    writeCommon();
+    if (!opt.output) return;
+    readForces();
+    if (timeStep == 1) forceFile << "timestep\tFx\tFy\n";
+    forceFile << timeStep * dt << '\t' << forceX << '\t' << forceY << std::endl;
+}
+
+// luo_base.cu:180-206
+void luo_base::writeCommon() {
+    NavierStokesSolver::writeCommon();
+    if (!opt.output) return;
+    if (timeStep % db->nsave == 0) writeBodies(outFolder(), timeStep, bX, bY);
+    if (timeStep == 1) midPositionFile << "ts" << "\t" << "X" << "\t" << "Y" << "\t" << "U" << "\t" << "V" << std::endl;
+    midPositionFile << timeStep << "\t" << midX << "\t" << midY << "\t" << bUB[0] << "\t" << bVB[0] << std::endl;
+}
+
+void luo_base::shutDown() {
+    NavierStokesSolver::shutDown();
+    if (forceFile.is_open()) forceFile.close();
+    if (midPositionFile.is_open()) midPositionFile.close();
+}
+
+DArray<int> *luo_base::ifield(const std::string &n) {
+    if (n == "ghostTagsUV") return &ghostTagsUV;
+    if (n == "hybridTagsUV") return &hybridTagsUV;
+    if (n == "hybridTagsUV2") return &hybridTagsUV2;
+    if (n == "ghostTagsP") return &ghostTagsP;
+    if (n == "hybridTagsP") return &hybridTagsP;
+    // per-thread records of the last tagging kernel (pressure nodes), for diagnostics
+    if (n == "tag_records_own") return &rec.own;
+    if (n == "tag_records_flags") return &rec.flags;
+    return nullptr;
+}
+
+DArray<double> *luo_base::ibmField(const std::string &n) {
+    if (n == "body_intercept_x") return &bi_x;
+    if (n == "body_intercept_y") return &bi_y;
+    if (n == "image_point_x") return &ip_x;
+    if (n == "image_point_y") return &ip_y;
+    if (n == "body_intercept_p_x") return &bip_x;
+    if (n == "body_intercept_p_y") return &bip_y;
+    if (n == "image_point_p_x") return &ipp_x;
+    if (n == "image_point_p_y") return &ipp_y;
+    if (n == "distance_from_intersection_to_node") return &dist_int;
+    if (n == "distance_between_nodes_at_IB") return &dist_between;
+    if (n == "distance_from_u_to_body") return &dist_u_body;
+    if (n == "distance_from_v_to_body") return &dist_v_body;
+    if (n == "uv") return &uvb;
+    if (n == "ustar") return &ustar;
+    if (n == "pstar") return &pstar;
+    if (n == "body_x") return &dbx;
+    if (n == "body_y") return &dby;
+    return nullptr;
+}
+
+long luo_base::fieldCount(const std::string &n) const {
+    if (auto *f = const_cast<luo_base *>(this)->ibmField(n)) return (long)f->n();
+    return NavierStokesSolver::fieldCount(n);
+}
+
+long luo_base::getField(const std::string &n, void *out, long cap) {
+    if (auto *f = ibmField(n)) {
+        CUIBM_HIP(hipStreamSynchronize(stream_));
+        const long c = std::min<long>(cap, (long)f->n());
+        f->download((double *)out, c, stream_);
+        return (long)f->n();
+    }
+    return NavierStokesSolver::getField(n, out, cap);
+}
+
+void luo_base::info(double *out, int n) const {
+    const_cast<luo_base *>(this)->readForces();
+    double v[32];
+    NavierStokesSolver::info(v, 32);
+    v[5] = forceX;
+    v[6] = forceY;
+    v[7] = fxx;
+    v[8] = fxy;
+    v[9] = fxu;
+    v[10] = startI;
+    v[11] = startJ;
+    v[12] = numCellsX;
+    v[13] = numCellsY;
+    v[14] = midX;
+    v[15] = midY;
+    v[21] = gnV.n;
+    v[22] = hnV.n;
+    v[23] = gnP.n;
+    v[24] = hnP.n;
+    v[25] = nsolid;
+    std::memcpy(out, v, sizeof(double) * std::min(n, 32));
+}
+
+int luo_base::phase(const std::string &n) {
+    if (n == "pre_step") _pre_step();
+    else if (n == "intermediate_velocity") _intermediate_velocity();
+    else if (n == "pressure_phase") _pressure();
+    else if (n == "project_velocity") _project_velocity();
+    else if (n == "update_body") _update_body();
+    else if (n == "post_step") _post_step();
+    else if (n == "tagPoints" || n == "tag") tagPoints();
+    else if (n == "calculateForce" || n == "force") calculateForce();
+    else if (n == "robin") updateRobinBoundary();
+    else return NavierStokesSolver::phase(n);
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+// ============================================================================ Luo external
+
+// luoIBM.cu:27-64
+void luoIBM::initialise() {
+    luo_base::initialise();
+    zeroVelocity();
+    initialiseLHS();
+}
+
+void luoIBM::initialiseLHS() {
+    generateLHS1();
+    generateLHS2();
+    buildPreconditioners();
+}
+
+void luoIBM::_intermediate_velocity() {
+    generateRHS1();
+    solveIntermediateVelocity();
+    weightUhat();
+}
+
+void luoIBM::_pressure() {
+    generateRHS2();
+    solvePoisson();
+    weightPressure();
+}
+
+// luoIBM/intermediateVelocity.cu:18-57 (u = uold is the identity here: uold was set from u)
+void luoIBM::generateRHS1() {
+    updateRobinBoundary();
+    rhs1GNInterpolation();
+    rhs1HNInterpolation();
+    launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
+                opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
+}
+
+void luoIBM::rhs1GNInterpolation() {
+    launch_gn_vel_apply(u.p(), gnV, numU, bUB[0], bVB[0], stream_);
+    zeroVelocity();
+}
+
+void luoIBM::rhs1HNInterpolation() { launch_hn_vel_apply(u.p(), ustar.p(), hnV, numU, bUB[0], bVB[0], stream_); }
+
+void luoIBM::zeroVelocity() { launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_); }
+
+void luoIBM::weightUhat() { launch_weight(uhat.p(), ustar.p(), hnV, stream_); }
+
+// luoIBM/intermediatePressure.cu:15-29, 79-103
+void luoIBM::preRHS2Interpolation() { launch_gn_vel_apply(uhat.p(), gnV, numU, bUB[0], bVB[0], stream_); }
+
+void luoIBM::generateRHS2() {
+    preRHS2Interpolation();
+    launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_);
+}
+
+// luoIBM/intermediatePressure.cu:44-77
+void luoIBM::weightPressure() {
+    const double dudt = (bUB[0] - uBk) / dt, dvdt = (bVB[0] - vBk) / dt;
+    launch_hn_p_apply(pressure.p(), pstar.p(), hnP, dudt, dvdt, stream_);
+    launch_weight(pressure.p(), pstar.p(), hnP, stream_);
+    launch_gn_p_apply(pressure.p(), gnP, dudt, dvdt, stream_);
+}
+
+int luoIBM::phase(const std::string &n) {
+    if (n == "weightUhat" || n == "weight_uhat") weightUhat();
+    else if (n == "weightPressure" || n == "weight_p") weightPressure();
+    else if (n == "rhs1GNInterpolation") rhs1GNInterpolation();
+    else if (n == "rhs1HNInterpolation" || n == "hn_u") rhs1HNInterpolation();
+    else if (n == "preRHS2Interpolation" || n == "gn_uhat") preRHS2Interpolation();
+    else if (n == "gn_u") launch_gn_vel_apply(u.p(), gnV, numU, bUB[0], bVB[0], stream_);
+    else if (n == "zeroVelocity" || n == "set_inside") zeroVelocity();
+    else return luo_base::phase(n);
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+// ============================================================================ modified Fadlun
+
+// fadlunModified.cu:28-59 (the matrices are assembled sorted, no sort needed)
+void fadlunModified::initialise() {
+    luo_base::initialise();
+    initialiseLHS();
+}
+
+void fadlunModified::initialiseLHS() {
+    generateLHS1();
+    generateLHS2();
+    buildPreconditioners();
+}
+
+FadlunRhsDev fadlunModified::fadlunRhs() const {
+    return FadlunRhsDev{hybridTagsUV.p(), ghostTagsUV.p(), dist_int.p(), dist_between.p(), uvb.p()};
+}
+FadlunP2Dev fadlunModified::fadlunP2() const {
+    return FadlunP2Dev{ghostTagsP.p(), hybridTagsP.p(), dist_u_body.p(), dist_v_body.p()};
+}
+FadlunLhsDev fadlunModified::fadlunLhs() const {
+    return FadlunLhsDev{ghostTagsUV.p(), hybridTagsUV.p(), hybridTagsUV2.p(), ghostTagsP.p(), hybridTagsP.p(),
+                        dist_int.p(), dist_between.p(), dist_u_body.p(), dist_v_body.p()};
+}
+
+// luo_base.cu:235-248: the Fadlun projection does not write uold, so copy it here
+void fadlunModified::_pre_step() {
+    luo_base::_pre_step();
+    launch_copy(uold.p(), u.p(), numUV, stream_);
+}
+
+void fadlunModified::_intermediate_velocity() {
+    generateRHS1();
+    solveIntermediateVelocity();
+}
+
+void fadlunModified::_pressure() {
+    generateRHS2();
+    solvePoisson();
+}
+
+// FadlunModified/intermediateVelocity.cu:15-38 (updateRHS1forIB fused into the sweep)
+void fadlunModified::generateRHS1() {
+    updateRobinBoundary();
+    launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
+                opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
+}
+
+void fadlunModified::generateRHS2() { launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_); }
+
+// fadlunModified.cu:109-124
+void fadlunModified::_project_velocity() {
+    launch_project(u.p(), uhat.p(), uold.p(), pressure.p(), grid(), dt, ghostTagsP.p(), stream_);
+}
+
+// fadlunModified.cu:64-95: the header test (timeStep == 0) is never true at write time
+void fadlunModified::writeData() {
+    writeCommon();
+    calculateForce();
+    if (!opt.output) return;
+    readForces();
+    forceFile << timeStep * dt << '\t' << forceX << '\t' << fxx << "\t" << fxy << "\t" << fxu << "\t" << forceY
+              << std::endl;
+}
+
+void fadlunModified::writeCommon() {
+    NavierStokesSolver::writeCommon();
+    if (!opt.output) return;
+    if (timeStep % db->nsave == 0) writeBodies(outFolder(), timeStep, bX, bY);
+}
+
+int fadlunModified::phase(const std::string &n) {
+    if (n == "fadlun_project") _project_velocity();
+    else return luo_base::phase(n);
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+// ============================================================================ factory
+
+std::unique_ptr<NavierStokesSolver> createSolver(ParameterDB *db, Domain *dom, const SolverOptions &opt) {
+    switch (db->solverType) {
+        case NAVIERSTOKES: return std::unique_ptr<NavierStokesSolver>(new NavierStokesSolver(db, dom, opt));
+        case FADLUN: return std::unique_ptr<NavierStokesSolver>(new fadlunModified(db, dom, opt));
+        case LUO: return std::unique_ptr<NavierStokesSolver>(new luoIBM(db, dom, opt));
+        case OSC: throw Error("OSC_CYLINDER (Luo embedded) is not supported yet", ERR_UNSUPPORTED);
+        default: throw Error("solver type FSI is not implemented in the reference either", ERR_UNSUPPORTED);
+    }
+}
+
+}  // namespace cuibm

@@ -1,0 +1,205 @@
+// Solver classes mirroring cuIBM's NavierStokesSolver / luo_base / luoIBM /
+// fadlunModified API (src/solvers/NavierStokes/NavierStokesSolver.h:145-186,
+// luo_base.h:216-243), driving the HIP kernels of this library.
+#pragma once
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "case_io.hpp"
+#include "ibm.hpp"
+#include "kernels.hpp"
+#include "krylov.hpp"
+
+namespace cuibm {
+
+struct SolverOptions {
+    bool output = true;          // write the reference's files into the output folder
+    std::string outputFolder;    // default: the case folder
+    bool keepDiagnostics = true; // store L and bc1 (not needed by the step itself)
+};
+
+class NavierStokesSolver {
+  public:
+    NavierStokesSolver(ParameterDB *pDB, Domain *dInfo, const SolverOptions &opt = SolverOptions());
+    virtual ~NavierStokesSolver();
+
+    virtual void initialise();
+    virtual void initialiseLHS();
+    virtual void stepTime();
+    virtual void writeCommon();
+    virtual void writeData();
+    bool finished() const { return timeStep >= db->nt; }
+    virtual void shutDown();
+
+    virtual void generateRHS1();
+    virtual void generateLHS1();
+    virtual void generateRHS2();
+    virtual void generateLHS2();
+    virtual void velocityProjection();
+    void solveIntermediateVelocity();
+    void solvePoisson();
+    virtual std::string name() const { return "Navier-Stokes"; }
+
+    // phase entry points used by the C ABI (reference method names where one exists)
+    virtual int phase(const std::string &name);
+
+    // field access in reference layouts; returns the element count or -1
+    virtual long fieldCount(const std::string &name) const;
+    virtual long getField(const std::string &name, void *out, long n);
+    virtual long setField(const std::string &name, const void *in, long n);
+    virtual void info(double *out, int n) const;
+
+    hipStream_t stream() const { return stream_; }
+    const SolveStat &lastSolve1() const { return stat1; }
+    const SolveStat &lastSolve2() const { return stat2; }
+    int step() const { return timeStep; }
+    // algorithmic HBM bytes of the last step's Poisson solve and whole step
+    double poissonBytes() const;
+    double velocityBytes() const;
+    double stepBytes() const;
+    std::string preconditionerName(int which) const;
+    // mean time of `reps` launches of one kernel on the live state (HIP events)
+    int timeKernel(const std::string &kernel, int reps, double *ms, double *bytes);
+    AmgHierarchyInfo amgInfo() const;
+
+  protected:
+    ParameterDB *db;
+    Domain *dom;
+    SolverOptions opt;
+    hipStream_t stream_ = nullptr;
+    int nx = 0, ny = 0, numU = 0, numV = 0, numUV = 0, numP = 0;
+    double nu = 0, dt = 0;
+    int timeStep = 0, iterationCount1 = 0, iterationCount2 = 0;
+    SolveStat stat1, stat2;
+
+    DArray<double> gx, gdx, gy, gdy, gxu, gyv;
+    DArray<double> u, uhat, uold, pressure, N, Nold, L, bc1, rhs1, rhs2, bc[4];
+    DCsr<double> LHS1, LHS2;
+    std::unique_ptr<Preconditioner> PC1, PC2;
+    std::unique_ptr<Bicgstab> ks1, ks2;
+    std::ofstream iterationsFile;
+
+    GridDev grid() const;
+    BcDev bcs() const;
+    std::string outFolder() const;
+    void initialiseNoBody();
+    void buildPreconditioners();
+    void projectPlain();
+    virtual FadlunRhsDev fadlunRhs() const { return FadlunRhsDev{nullptr, nullptr, nullptr, nullptr, nullptr}; }
+    virtual FadlunP2Dev fadlunP2() const { return FadlunP2Dev{nullptr, nullptr, nullptr, nullptr}; }
+    virtual FadlunLhsDev fadlunLhs() const {
+        return FadlunLhsDev{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    }
+    virtual bool centrePin() const { return true; }
+    DArray<double> *dfield(const std::string &name);
+    const DArray<double> *dfield(const std::string &name) const;
+    virtual DArray<int> *ifield(const std::string &name) { return nullptr; }
+};
+
+// common immersed-boundary layer (luo_base.h:13-244)
+class luo_base : public NavierStokesSolver {
+  public:
+    luo_base(ParameterDB *pDB, Domain *dInfo, const SolverOptions &opt = SolverOptions());
+    void initialise() override;
+    void initialiseLHS() override {}
+    void writeData() override;
+    void writeCommon() override;
+    void shutDown() override;
+    void stepTime() override;
+    virtual void _pre_step();
+    virtual void _intermediate_velocity() {}
+    virtual void _pressure() {}
+    virtual void _project_velocity();
+    virtual void _update_body();
+    virtual void _post_step();
+    void moveBody();
+    void updateSolver();
+    void tagPoints();
+    void calculateForce();
+    void updateRobinBoundary();
+
+    int phase(const std::string &name) override;
+    long fieldCount(const std::string &name) const override;
+    long getField(const std::string &name, void *out, long n) override;
+    void info(double *out, int n) const override;
+
+  protected:
+    // body state (src/bodies.h)
+    std::vector<double> bX, bY, bUB, bVB;  // host copies of the body points / velocities
+    double midX = 0, midY = 0, uB0 = 0, vB0 = 0, uBk = 0, vBk = 0;
+    int startI = 0, startJ = 0, numCellsX = 0, numCellsY = 0;
+    double forceX = 0, forceY = 0, fxx = 0, fxy = 0, fxu = 0;
+    DArray<double> dbx, dby, dbu, dbv;
+    // tags and intercept data (luo_base/cast.cu:7-82)
+    DArray<int> ghostTagsUV, hybridTagsUV, hybridTagsUV2, ghostTagsP, hybridTagsP;
+    DArray<double> bi_x, bi_y, ip_x, ip_y, bip_x, bip_y, ipp_x, ipp_y;
+    DArray<double> dist_int, dist_between, dist_u_body, dist_v_body, uvb;
+    DArray<double> ustar, pstar;
+    DArray<double> forcePart, forceOut;
+    TagRecords rec;
+    Stencils gnV, hnV, hnP, gnP;
+    DArray<int> solid;
+    int nsolid = 0;
+    std::ofstream forceFile, midPositionFile;
+    bool tagged = false;
+
+    void boundingBox();
+    void buildStencils();
+    void readForces();
+    DArray<int> *ifield(const std::string &name) override;
+    const DArray<int> *ifieldc(const std::string &name) const;
+    DArray<double> *ibmField(const std::string &name);
+};
+
+// Luo et al. external ghost-node method (luoIBM.h)
+class luoIBM : public luo_base {
+  public:
+    using luo_base::luo_base;
+    void initialise() override;
+    void initialiseLHS() override;
+    void _intermediate_velocity() override;
+    void _pressure() override;
+    void generateRHS1() override;
+    void generateRHS2() override;
+    void weightUhat();
+    void weightPressure();
+    void rhs1GNInterpolation();
+    void rhs1HNInterpolation();
+    void preRHS2Interpolation();
+    void zeroVelocity();
+    std::string name() const override { return "LUO"; }
+    int phase(const std::string &name) override;
+
+  protected:
+    bool centrePin() const override { return false; }
+};
+
+// modified Fadlun direct forcing (fadlunModified.h)
+class fadlunModified : public luo_base {
+  public:
+    using luo_base::luo_base;
+    void initialise() override;
+    void initialiseLHS() override;
+    void writeData() override;
+    void writeCommon() override;
+    void _pre_step() override;
+    void _intermediate_velocity() override;
+    void _pressure() override;
+    void _project_velocity() override;
+    void generateRHS1() override;
+    void generateRHS2() override;
+    std::string name() const override { return "FADLUN"; }
+    int phase(const std::string &name) override;
+
+  protected:
+    FadlunRhsDev fadlunRhs() const override;
+    FadlunP2Dev fadlunP2() const override;
+    FadlunLhsDev fadlunLhs() const override;
+};
+
+// factory, the switch of src/cuIBM.cu:42-60
+std::unique_ptr<NavierStokesSolver> createSolver(ParameterDB *db, Domain *dom, const SolverOptions &opt);
+
+}  // namespace cuibm

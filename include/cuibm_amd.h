@@ -1,0 +1,161 @@
+/*
+ * cuibm_amd — C ABI of the MI355X (gfx950) rebuild of cuIBM's per-time-step hot path.
+ *
+ * The reference has no FFI: its hot path sits behind two C++ interfaces, the
+ * solver-class API (src/solvers/NavierStokes/NavierStokesSolver.h:145-186,
+ * luo_base.h:216-243, the factory switch in src/cuIBM.cu:42-60 and the driver loop
+ * src/cuIBM.cu:62-74) and the CUSP linear-operator API
+ * (cusp::krylov::bicgstab(A, x, b, monitor, M) at NavierStokesSolver.cu:227-228 and
+ * :256-257, preconditioner<coo_matrix> in src/preconditioner.h:35-195). Each entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions: every function returns 0 (or a count) on success and a negative
+ * code on failure (the reference calls exit(-1) instead); cuibm_last_error() gives
+ * the message. The library owns device memory, callers own host buffers. One HIP
+ * stream per solver; a solver handle is not thread-safe. Field layouts are the
+ * reference's (packed u then v velocities, iu = J*(nx-1)+I, iv = numU + J*nx+I;
+ * pressure ip = J*nx+I).
+ */
+#ifndef CUIBM_AMD_H
+#define CUIBM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define CUIBM_OK 0
+#define CUIBM_ERR_GENERIC (-1)
+#define CUIBM_ERR_INPUT (-2)
+#define CUIBM_ERR_HIP (-3)
+#define CUIBM_ERR_NOT_CONVERGED (-4)  /* NavierStokesSolver.cu:232-240, 260-268 exit(-1) */
+#define CUIBM_ERR_UNSUPPORTED (-5)
+#define CUIBM_ERR_ARGUMENT (-6)
+
+typedef struct cuibm_solver cuibm_solver;
+typedef struct cuibm_csr cuibm_csr;
+
+/* library version (major*10000 + minor*100 + patch) */
+int cuibm_version(void);
+
+/* message of the last failed call on this thread */
+const char *cuibm_last_error(void);
+
+/* ---- solver lifecycle ----------------------------------------------------------- */
+
+/* io::readInputs (src/io/io.cu:108-138) + the SolverType factory (src/cuIBM.cu:42-60):
+ * read <case_folder>/{simParams,flow,domain,bodies}.yaml, apply the command-line
+ * overrides in argv (-nu -uPerturb -vPerturb -scaleCV -nsave -nt -dt -velocityTol
+ * -poissonTol, plus -amgPrecision 32|64, -poissonPreconditioner NAME,
+ * -outputFolder DIR and -output 0|1) and construct the solver (not yet initialised). */
+int cuibm_create(const char *case_folder, int argc, const char *const *argv, cuibm_solver **out);
+
+/* NavierStokesSolver::initialise() (NavierStokesSolver.cu:34-40, luoIBM.cu:27-52,
+ * fadlunModified.cu:28-43): fields, grid, bodies, tags, matrices, preconditioners */
+int cuibm_initialise(cuibm_solver *s);
+
+/* NavierStokesSolver::stepTime() (NavierStokesSolver.cu:163-188, luo_base.cu:208-233) */
+int cuibm_step_time(cuibm_solver *s);
+
+/* NavierStokesSolver::writeData() (NavierStokesSolver.cu:356-363, luoIBM.cu:69-80,
+ * fadlunModified.cu:64-77): q/lambda every nsave steps, iterations, forces */
+int cuibm_write_data(cuibm_solver *s);
+
+/* NavierStokesSolver::finished() (NavierStokesSolver.cu:195-199): 1 when done, 0 if not */
+int cuibm_finished(cuibm_solver *s);
+
+/* NavierStokesSolver::shutDown() (NavierStokesSolver.cu:368-372) */
+int cuibm_shut_down(cuibm_solver *s);
+
+/* release the solver and its device memory */
+void cuibm_destroy(cuibm_solver *s);
+
+/* NavierStokesSolver::name() */
+int cuibm_name(cuibm_solver *s, char *buf, int len);
+
+/* ---- case front end (host only; valid right after cuibm_create, no GPU needed) --- */
+
+/* parsed case arrays (src/domain.h:17-34, src/io/parseBodiesFile.cu:145-178): "x", "dx",
+ * "y", "dy", "xu", "yv" (grid) and "body_X", "body_Y" (body points as read, before
+ * re-centring). Copies at most n values; returns the array length. */
+long cuibm_case_array(cuibm_solver *s, const char *name, double *out, long n);
+
+/* parsed case scalars: out[0] nx, [1] ny, [2] nu, [3] dt, [4] nt, [5] nsave,
+ * [6] SolverType (src/types.h:4-11), [7] scaleCV, [8] startStep, [9..10] velocity
+ * and Poisson tolerance, [11..12] their maxIterations, [13..14] their preconditioner
+ * (src/preconditioner.h:23-29), [15] AMG precision, [16] uInitial, [17] vInitial,
+ * [18] uPerturb, [19] vPerturb, [20..27] boundary values u,v for xMinus, xPlus,
+ * yMinus, yPlus, [28] mid_h, [29] number of bodies */
+int cuibm_case_scalars(cuibm_solver *s, double *out, int n);
+
+/* ---- phase entry points (parity tests) ------------------------------------------- */
+
+/* run one named phase of the step on the solver's stream and synchronise. Names are
+ * the reference methods: generateRHS1, solveIntermediateVelocity, generateRHS2,
+ * solvePoisson, velocityProjection, generateLHS1, generateLHS2; for the IBM solvers
+ * also pre_step, intermediate_velocity, pressure_phase, project_velocity,
+ * update_body, post_step, tagPoints, calculateForce, robin and (LUO) weightUhat,
+ * weightPressure, rhs1GNInterpolation, rhs1HNInterpolation, preRHS2Interpolation,
+ * zeroVelocity. */
+int cuibm_phase(cuibm_solver *s, const char *name);
+
+/* ---- field access in reference layouts --------------------------------------------- */
+
+/* element count of a field, or a negative code. Double fields: u, uhat, uold, p,
+ * N, Nold, L, bc1, rhs1, rhs2, xm, xp, ym, yp, x, dx, y, dy, xu, yv, LHS1_val,
+ * LHS2_val, ustar, pstar, body_intercept_{x,y}, image_point_{x,y},
+ * body_intercept_p_{x,y}, image_point_p_{x,y}, distance_from_intersection_to_node,
+ * distance_between_nodes_at_IB, distance_from_{u,v}_to_body, uv, body_{x,y}.
+ * Int fields: ghostTagsUV, hybridTagsUV, hybridTagsUV2, ghostTagsP, hybridTagsP,
+ * LHS{1,2}_rowptr, LHS{1,2}_col. */
+long cuibm_field_size(cuibm_solver *s, const char *name);
+/* copy a field to host memory (at most n elements); returns the field's element count */
+long cuibm_get_field(cuibm_solver *s, const char *name, void *host, long n);
+/* overwrite a field from host memory (n must equal its size) */
+long cuibm_set_field(cuibm_solver *s, const char *name, const void *host, long n);
+
+/* scalars: out[0] timeStep, [1..2] iterationCount1/2, [3..4] final residual norms,
+ * [5..6] forceX/forceY, [7..9] FxX/FxY/FxU, [10..13] startI/startJ/numCellsX/Y,
+ * [14..15] midX/midY, [16] AMG levels, [17..18] nx/ny, [19..20] solver tolerances,
+ * [21..25] ghost/hybrid velocity, ghost/hybrid pressure and solid node counts */
+int cuibm_info(cuibm_solver *s, double *out, int n);
+
+/* algorithmic HBM bytes of the last step: which = 0 whole step, 1 velocity solve,
+ * 2 Poisson solve */
+double cuibm_bytes(cuibm_solver *s, int which);
+
+/* ---- timing (HIP events on the solver's stream) ----------------------------------- */
+
+/* run nsteps time steps and return their elapsed device time in *ms */
+int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms);
+
+/* time `reps` back-to-back launches of one kernel of the step on its live inputs and
+ * return the mean launch time in *ms and the algorithmic bytes per launch in *bytes.
+ * kernel: "spmv_lhs2" (the Poisson operator product y = LHS2 x), "spmv_lhs1",
+ * "rhs1" (fused intermediate-velocity right-hand side), "vcycle" (one Poisson
+ * preconditioner application) */
+int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms, double *bytes);
+
+/* ---- linear algebra (replaces cusp::krylov::bicgstab / cusp::precond) ------------- */
+
+/* upload a CSR matrix (n x n, 0-based, sorted columns) */
+int cuibm_csr_create(int n, const int *rowptr, const int *col, const double *val, cuibm_csr **out);
+void cuibm_csr_destroy(cuibm_csr *A);
+/* y = A x (host vectors) */
+int cuibm_csr_spmv(cuibm_csr *A, const double *x, double *y);
+/* cusp::krylov::bicgstab(A, x, b, cusp::monitor(b, maxit, rtol), M): x (host) is the
+ * warm start and the result; precond = 0 NONE, 1 DIAGONAL, 2 SMOOTHED_AGGREGATION
+ * (preconditioner.h:23-29); amg_precision = 64 or 32. Returns 0 when converged,
+ * CUIBM_ERR_NOT_CONVERGED otherwise, with *iters and *resid set either way. */
+int cuibm_bicgstab(cuibm_csr *A, double *x, const double *b, int precond, int amg_precision, double rtol, int maxit,
+                   int *iters, double *resid);
+/* smoothed-aggregation hierarchy of A: level sizes and nonzeros; returns the level count */
+int cuibm_amg_levels(cuibm_csr *A, int *sizes, int *nnz, int cap);
+/* one V-cycle x = M^-1 b from a zero guess (host vectors) */
+int cuibm_amg_vcycle(cuibm_csr *A, int amg_precision, const double *b, double *x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUIBM_AMD_H */

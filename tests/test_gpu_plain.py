@@ -1,0 +1,59 @@
+"""Plain Navier-Stokes step on the GPU against the oracle (lid-driven cavity).
+
+Parity bar: per-step relative L2 of u and p within 1e-10 (solver tolerances are
+tightened so that the BiCGSTAB stopping point does not dominate the difference).
+"""
+import numpy as np
+import pytest
+from helpers import pair, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = {"velocityTol": 1e-12, "poissonTol": 1e-12}
+
+
+def test_matrices_match_oracle():
+    g, o = pair("lidDrivenCavityRe100_32", TIGHT)
+    for m in ("LHS1", "LHS2"):
+        assert np.array_equal(g.get(m + "_rowptr"), o.get(m + "_rowptr"))
+        assert np.array_equal(g.get(m + "_col"), o.get(m + "_col"))
+        assert rel_l2(g.get(m + "_val"), o.get(m + "_val")) < 1e-15
+
+
+def test_rhs1_pieces_match_oracle():
+    g, o = pair("lidDrivenCavityRe100_32", TIGHT)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal(g.get("u").size)
+    nold = rng.standard_normal(u.size)
+    for s in (g, o):
+        s.set("u", u)
+        s.set("Nold", nold)
+    g.phase("rhs1_only")
+    for ph in ("generateN", "generateL", "generateBC1", "generateRHS"):
+        o.phase(ph)
+    for f in ("N", "L", "bc1", "rhs1"):
+        assert rel_l2(g.get(f), o.get(f)) < 1e-14, f
+
+
+@pytest.mark.parametrize("name", ["lidDrivenCavityRe100_32", "lidDrivenCavityRe100_32_jacobi"])
+def test_steps_match_oracle(name):
+    g, o = pair(name, TIGHT)
+    for k in range(10):
+        g.step_time()
+        o.step()
+        for f in ("u", "p"):
+            e = rel_l2(g.get(f), o.get(f))
+            assert e < 1e-10, (k, f, e)
+    gi, oi = g.info(), o.info()
+    assert gi["timeStep"] == oi["timeStep"] == 10
+
+
+def test_cavity128_cfg1_100_steps():
+    """The 128x128 Re=100 cavity (SA-AMG Poisson, rtol 1e-8 as configured)."""
+    g, o = pair("lidDrivenCavityRe100_128")
+    for k in range(5):
+        g.step_time(20)
+        o.step(20)
+        for f in ("u", "p"):
+            e = rel_l2(g.get(f), o.get(f))
+            assert e < 1e-10, (k, f, e)
