@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Time-step throughput of the cuIBM per-step hot path on one MI355X.
+
+Workload: impulsively started circular cylinder at Re=550 (case
+``cuibm_amd/cases/cylinderRe550_1024_luo``): D=1 at the origin, [-15,15]^2 with
+h=0.005 on [-0.6,0.6]^2 stretched by 1.008178 to 1024x1024 cells, Luo external
+ghost-node IBM with 628 body points, dt=0.0025. Velocity solve: BiCGSTAB + Jacobi,
+rtol 1e-9. Poisson solve: FP64 BiCGSTAB, rtol 1e-9, preconditioned by the
+smoothed-aggregation V-cycle (2+2 Jacobi sweeps, omega 0.8) whose matrix values are
+stored in FP32 while its vectors and arithmetic stay FP64 (the headline "mixed"
+form); the same steps are also run with an FP64 hierarchy and the drag coefficients
+of the two forms are compared.
+
+A step is one full ``stepTime`` (rhs1, velocity solve, rhs2, Poisson solve, IBM
+interpolation/blending, projection, control-volume force) on fields resident in
+HBM. W untimed warm-up steps are followed by exactly K timed steps, bracketed by
+device synchronisation and measured with HIP events on the solver's stream.
+
+Prints one JSON line. ``roofline`` reports the dominant kernel of the step (the
+Poisson operator product fused with the BiCGSTAB dot partials) timed with HIP
+events over back-to-back launches on the live Poisson matrix; ``traffic`` is the
+HBM byte count per launch of that kernel from hardware counters, read from the
+committed summary under profiles/ when present. ``cpu_baseline`` is the FP64 C++
+CPU restatement of the same step (the oracle) timed on this host over a short
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("time steps/s + HBM GB/s at 1024² (mixed vs FP64, same C_D accuracy); adjoint s/grad")
+CASE = "cylinderRe550_1024_luo"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_spmv_dot_lhs2_cylinder1024.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(cu, case, amg_precision, warmup, steps, kernel, reps):
+    """initialise, warm up, time `steps` steps; return a dict of figures"""
+    t0 = time.perf_counter()
+    s = cu.Solver(case, "-amgPrecision", str(amg_precision), diagnostics=False)
+    s.initialise()
+    t_init = time.perf_counter() - t0
+    if warmup > 0:
+        s.step_time(warmup)
+    i0 = s.info()
+    cu.device_synchronize()
+    w0 = time.perf_counter()
+    ms = s.time_steps(steps)
+    cu.device_synchronize()
+    wall = time.perf_counter() - w0
+    i1 = s.info()
+    d = lambda k: i1[k] - i0[k]
+    out = {
+        "amg_precision": amg_precision,
+        "init_s": round(t_init, 3),
+        "ms_per_step": ms / steps,
+        "wall_ms_per_step": 1e3 * wall / steps,
+        "steps_per_s": 1e3 * steps / ms,
+        "velocity_iters_per_step": d("iter1_total") / steps,
+        "poisson_iters_per_step": d("iter2_total") / steps,
+        "poisson_bytes_per_step": d("poisson_bytes_total") / steps,
+        "velocity_bytes_per_step": d("velocity_bytes_total") / steps,
+        "cd": 2.0 * i1["forceX"],
+        "cl": 2.0 * i1["forceY"],
+        "time_step": int(i1["timeStep"]),
+    }
+    # step bytes = the two solves (counted per iteration as run) + the stencil sweeps
+    out["step_bytes"] = s.bytes(0) - s.bytes(1) - s.bytes(2) + out["poisson_bytes_per_step"] + \
+        out["velocity_bytes_per_step"]
+    out["hbm_gbps_step"] = out["step_bytes"] / (out["ms_per_step"] * 1e-3) / 1e9
+    if kernel:
+        kms, kbytes = s.time_kernel(kernel, reps)
+        out["kernel"] = {"name": kernel, "ms": kms, "bytes": kbytes}
+    s.close()
+    return out
+
+
+def read_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed counter summary"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def cpu_baseline(case_dir, steps):
+    """time the FP64 oracle (the CPU restatement of the same step) on this host"""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle as po
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    c = po.read_case(case_dir)  # the oracle's hierarchy is always FP64
+    o = po.Oracle(c)
+    o.step(1)  # warm-up step (first-touch of the work arrays)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": steps / dt, "unit": "time steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} time steps (after 1 warm-up step) of the same 1024x1024 Luo cylinder case with "
+                      f"the FP64 C++ oracle (FP64 SA-AMG), OpenMP threads={threads}; "
+                      f"{dt:.1f} s of CPU work; initialisation excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--kernel", default="spmv_dot_lhs2", help="roofline kernel (see cuibm_time_kernel)")
+    ap.add_argument("--kernel-reps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-hierarchy comparison run")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="initialise the case and launch only the roofline kernel and the copy calibration kernel "
+                         "(for hardware-counter runs); prints their HIP-event timings")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only: the 1024x1024 step runs on a single MI355X "
+              f"(--gpus {args.gpus}, WORLD_SIZE {world} requested); multi-GPU sharding is not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cuibm_amd as cu
+
+    case = os.path.join(cu.CASES, CASE)
+    if args.kernels_only:
+        s = cu.Solver(case, diagnostics=False)
+        s.initialise()
+        out = {}
+        for k in (args.kernel, "copy"):
+            ms, b = s.time_kernel(k, args.kernel_reps)
+            out[k] = {"us": ms * 1e3, "bytes": b, "gbps": b / (ms * 1e-3) / 1e9}
+        s.close()
+        print(json.dumps(out), flush=True)
+        return
+    log(f"[bench] mixed form (FP32 AMG matrix values): warmup {args.warmup}, steps {args.steps}")
+    mixed = run_form(cu, case, 32, args.warmup, args.steps, args.kernel, args.kernel_reps)
+    log(f"[bench] mixed: {mixed['steps_per_s']:.2f} steps/s, Poisson its/step {mixed['poisson_iters_per_step']:.1f}")
+    fp64 = None
+    if not args.no_fp64:
+        log("[bench] FP64 form")
+        fp64 = run_form(cu, case, 64, args.warmup, args.steps, None, 0)
+        log(f"[bench] fp64: {fp64['steps_per_s']:.2f} steps/s, Poisson its/step {fp64['poisson_iters_per_step']:.1f}")
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
+        cpu = cpu_baseline(case, args.cpu_steps)
+        log(f"[bench] cpu: {cpu['value']:.3f} steps/s")
+
+    k = mixed["kernel"]
+    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(mixed["steps_per_s"], 4),
+        "unit": "time steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "cylinderRe550_1024_luo: Re=550 cylinder, 1024x1024 stretched MAC grid, Luo external "
+                        "IBM, dt=0.0025; Poisson FP64 BiCGSTAB rtol 1e-9 + SA-AMG V-cycle with FP32 matrix values "
+                        "(FP64 vectors and arithmetic); velocity FP64 BiCGSTAB+Jacobi rtol 1e-9",
+            "grid": "1024x1024",
+            "time_step_range": [mixed["time_step"] - args.steps + 1, mixed["time_step"]],
+        },
+        "hbm_gbps_step": round(mixed["hbm_gbps_step"], 1),
+        "poisson_gbps": round(mixed["poisson_bytes_per_step"] / (mixed["ms_per_step"] * 1e-3) / 1e9, 1),
+        "poisson_iters_per_step": mixed["poisson_iters_per_step"],
+        "velocity_iters_per_step": mixed["velocity_iters_per_step"],
+        "cd": mixed["cd"],
+        "cl": mixed["cl"],
+        "roofline": {
+            "kernel": k["name"],
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(),
+            "bytes_per_launch": k["bytes"],
+            "us_per_launch": round(k["ms"] * 1e3, 2),
+        },
+        "cpu_baseline": cpu,
+    }
+    if fp64:
+        line["fp64"] = {
+            "steps_per_s": round(fp64["steps_per_s"], 4),
+            "ms_per_step": round(fp64["ms_per_step"], 4),
+            "poisson_iters_per_step": fp64["poisson_iters_per_step"],
+            "hbm_gbps_step": round(fp64["hbm_gbps_step"], 1),
+            "cd": fp64["cd"],
+            "cl": fp64["cl"],
+        }
+        line["speedup_mixed_vs_fp64"] = round(mixed["steps_per_s"] / fp64["steps_per_s"], 4)
+        line["abs_dcd_mixed_vs_fp64"] = abs(mixed["cd"] - fp64["cd"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

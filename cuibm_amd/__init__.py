@@ -32,7 +32,9 @@ INT_FIELDS = {"ghostTagsUV", "hybridTagsUV", "hybridTagsUV2", "ghostTagsP", "hyb
               "LHS1_rowptr", "LHS1_col", "LHS2_rowptr", "LHS2_col", "tag_records_own", "tag_records_flags"}
 INFO_KEYS = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "fxx", "fxy", "fxu",
              "startI", "startJ", "numCellsX", "numCellsY", "midX", "midY", "amg_levels", "nx", "ny",
-             "tol1", "tol2", "n_ghost_uv", "n_hybrid_uv", "n_ghost_p", "n_hybrid_p", "n_solid"]
+             "tol1", "tol2", "n_ghost_uv", "n_hybrid_uv", "n_ghost_p", "n_hybrid_p", "n_solid",
+             "iter1_total", "iter2_total", "velocity_bytes_total", "poisson_bytes_total", "poisson_solves",
+             "krylov_restarts"]
 
 
 class CuibmError(RuntimeError):
@@ -75,6 +77,8 @@ def lib():
     L.cuibm_set_field.restype = C.c_long
     L.cuibm_info.argtypes = [C.c_void_p, _DP, C.c_int]
     L.cuibm_bytes.argtypes = [C.c_void_p, C.c_int]
+    L.cuibm_device_synchronize.argtypes = []
+    L.cuibm_device_synchronize.restype = C.c_int
     L.cuibm_bytes.restype = C.c_double
     L.cuibm_time_steps.argtypes = [C.c_void_p, C.c_int, _DP]
     L.cuibm_time_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _DP, _DP]
@@ -93,6 +97,11 @@ def _check(rc):
     if rc < 0:
         raise CuibmError(rc, lib().cuibm_last_error().decode(errors="replace"))
     return rc
+
+
+def device_synchronize():
+    """wait for all work on the current device (hipDeviceSynchronize)"""
+    _check(lib().cuibm_device_synchronize())
 
 
 def _dp(a):

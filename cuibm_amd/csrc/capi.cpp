@@ -7,6 +7,7 @@
 
 #include "../../include/cuibm_amd.h"
 #include "solver.hpp"
+#include "yaml_lite.hpp"
 
 using namespace cuibm;
 
@@ -39,6 +40,9 @@ int guard(F &&f) {
     } catch (const Error &e) {
         g_err = e.what();
         return e.code;
+    } catch (const YamlError &e) {  // unreadable or malformed case files
+        g_err = e.what();
+        return CUIBM_ERR_INPUT;
     } catch (const std::exception &e) {
         g_err = e.what();
         return CUIBM_ERR_GENERIC;
@@ -52,6 +56,9 @@ long guardl(F &&f) {
     } catch (const Error &e) {
         g_err = e.what();
         return e.code;
+    } catch (const YamlError &e) {  // unreadable or malformed case files
+        g_err = e.what();
+        return CUIBM_ERR_INPUT;
     } catch (const std::exception &e) {
         g_err = e.what();
         return CUIBM_ERR_GENERIC;
@@ -258,6 +265,13 @@ int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms) {
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         if (ms) *ms = t;
+        return 0;
+    });
+}
+
+int cuibm_device_synchronize(void) {
+    return guard([&] {
+        CUIBM_HIP(hipDeviceSynchronize());
         return 0;
     });
 }

@@ -167,7 +167,7 @@ void parseBodies(const YamlNode &doc, const std::string &folder, ParameterDB &DB
         if (type == "points") {
             const std::string fname = folder + "/bodyFiles/" + node["pointsFile"].as_string();
             std::ifstream f(fname);
-            if (!f) throw std::runtime_error("cannot open body points file " + fname);
+            if (!f) throw YamlError("cannot open body points file " + fname);
             f >> b.numPoints;
             b.X.resize(b.numPoints);
             b.Y.resize(b.numPoints);

@@ -82,10 +82,10 @@ void launch_to_double(double *y, const float *x, long n, hipStream_t s) {
 template <class V, class X>
 __device__ __forceinline__ X row_dot(const int *__restrict__ rp, const int *__restrict__ ci, const V *__restrict__ v,
                                      const X *__restrict__ x, int i) {
-    X s = 0;
+    double s = 0;  // FP64 accumulation also for float operands
     const int e = rp[i + 1];
-    for (int k = rp[i]; k < e; k++) s += (X)v[k] * x[ci[k]];
-    return s;
+    for (int k = rp[i]; k < e; k++) s += (double)v[k] * (double)x[ci[k]];
+    return (X)s;
 }
 
 template <class V, class X>
@@ -101,7 +101,7 @@ void launch_spmv(const DCsr<V> &A, const X *x, X *y, hipStream_t s) {
     CUIBM_LAUNCH_CHECK();
 }
 template void launch_spmv<double, double>(const DCsr<double> &, const double *, double *, hipStream_t);
-template void launch_spmv<float, float>(const DCsr<float> &, const float *, float *, hipStream_t);
+template void launch_spmv<float, double>(const DCsr<float> &, const double *, double *, hipStream_t);
 
 // y = A x with block partials of (y . a) and, when b2 is set, (y . y)
 __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int *__restrict__ rp, const int *__restrict__ ci,
@@ -267,9 +267,11 @@ __global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__
         case FIN_SS:
             sc[SC_SS] = tot[0];
             break;
-        case FIN_OMEGA:  // (AMs.s, AMs.AMs)
-            sc[SC_OMEGA] = tot[0] / tot[1];
+        case FIN_OMEGA: {  // (AMs.s, AMs.AMs); a breakdown leaves omega = 0 so that x stays finite
+            const double w = tot[0] / tot[1];
+            sc[SC_OMEGA] = isfinite(w) ? w : 0.0;
             break;
+        }
         case FIN_BETA: {  // (rs.r, r.r)
             sc[SC_BETA] = (tot[0] / sc[SC_RHO]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
             sc[SC_RHO] = tot[0];

@@ -5,8 +5,15 @@
 // ||r||_2 <= rtol ||b||_2 checked on r before every iteration and on s half-way
 // through it; the iteration count counts full iterations. The recurrence scalars
 // (rho, alpha, omega, beta) never leave the GPU: reductions finish in a one-block
-// kernel that updates them in place, and the host reads back only the two residual
-// norms it needs for the stopping test (two 8-byte copies per iteration).
+// kernel that updates them in place, and the host reads back the scalar block twice
+// per iteration for the stopping test.
+//
+// Breakdown (alpha or ||s|| not finite, omega = 0, rho = 0, beta not finite) restarts
+// the recurrence from the true residual b - A x of the current iterate, keeping the
+// iteration count; x never takes a non-finite update (omega falls back to 0). This
+// never triggers in a run whose recurrence stays finite, so such runs follow the
+// textbook iteration exactly; it matters for preconditioners that are not exactly
+// linear (FP32 storage), where a breakdown would otherwise spin to maxit on NaN.
 #include <cmath>
 
 #include "krylov.hpp"
@@ -42,13 +49,19 @@ SolveStat Bicgstab::solve(const DCsr<double> &A, double *x, const double *b, Pre
         return host_sc_[slot];
     };
     SolveStat st;
-    launch_bicg_init(A, x, b, r_.p(), p_.p(), rs_.p(), part, s);
-    launch_bicg_finish(part, FIN_INIT, sc, s);
-    CUIBM_HIP(hipMemcpyAsync(host_sc_, sc, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
-    CUIBM_HIP(hipStreamSynchronize(s));
+    auto fetch = [&]() {  // the whole scalar block: one small copy, one sync
+        CUIBM_HIP(hipMemcpyAsync(host_sc_, sc, SC_D0 * sizeof(double), hipMemcpyDeviceToHost, s));
+        CUIBM_HIP(hipStreamSynchronize(s));
+    };
+    auto start = [&]() {  // r = b - A x, rs = p = r
+        launch_bicg_init(A, x, b, r_.p(), p_.p(), rs_.p(), part, s);
+        launch_bicg_finish(part, FIN_INIT, sc, s);
+        fetch();
+    };
+    start();
     const double tol = rtol * std::sqrt(host_sc_[SC_BB]);
     double rn = std::sqrt(host_sc_[SC_RR]);
-    int it = 0;
+    int it = 0, restarts = 0;
     st.tolerance = tol;
     while (!(rn <= tol) && it < maxit) {
         M.apply(p_.p(), Mp_.p(), s);
@@ -56,7 +69,15 @@ SolveStat Bicgstab::solve(const DCsr<double> &A, double *x, const double *b, Pre
         launch_bicg_finish(part, FIN_ALPHA, sc, s);
         launch_bicg_s(n, r_.p(), AMp_.p(), s_.p(), sc, part, s);
         launch_bicg_finish(part, FIN_SS, sc, s);
-        rn = std::sqrt(read(SC_SS));
+        fetch();
+        const double ss = host_sc_[SC_SS];
+        if (!std::isfinite(ss) || !std::isfinite(host_sc_[SC_ALPHA])) {
+            if (++restarts > kMaxRestarts) break;
+            start();  // x is untouched in this half-iteration
+            rn = std::sqrt(host_sc_[SC_RR]);
+            continue;
+        }
+        rn = std::sqrt(ss);
         if (rn <= tol) {
             launch_bicg_xhalf(n, x, Mp_.p(), sc, s);
             break;
@@ -68,8 +89,16 @@ SolveStat Bicgstab::solve(const DCsr<double> &A, double *x, const double *b, Pre
         launch_bicg_finish(part, FIN_BETA, sc, s);
         launch_bicg_p(n, p_.p(), r_.p(), AMp_.p(), sc, s);
         it++;
-        rn = std::sqrt(read(SC_RR));
+        fetch();
+        rn = std::sqrt(host_sc_[SC_RR]);
+        if (!(rn <= tol) && (host_sc_[SC_OMEGA] == 0.0 || host_sc_[SC_RHO] == 0.0 ||
+                             !std::isfinite(host_sc_[SC_BETA]) || !std::isfinite(rn))) {
+            if (++restarts > kMaxRestarts) break;
+            start();
+            rn = std::sqrt(host_sc_[SC_RR]);
+        }
     }
+    st.restarts = restarts;
     st.iterations = it;
     st.residual = rn;
     st.converged = rn <= tol;

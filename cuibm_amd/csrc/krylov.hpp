@@ -109,6 +109,7 @@ std::vector<double> dense_inverse(const HCsr &A);
 // ---- BiCGSTAB (krylov.hip)
 struct SolveStat {
     int iterations = 0;
+    int restarts = 0;  // breakdown restarts (0 in a recurrence that stays finite)
     double residual = 0, tolerance = 0;
     bool converged = false;
 };
@@ -123,6 +124,7 @@ class Bicgstab {
     int n() const { return n_; }
 
   private:
+    static constexpr int kMaxRestarts = 64;
     int n_;
     DArray<double> r_, rs_, p_, Mp_, AMp_, s_, Ms_, AMs_, part_, sc_;
     double *host_sc_ = nullptr;

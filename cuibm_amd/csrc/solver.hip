@@ -201,6 +201,9 @@ void NavierStokesSolver::solveIntermediateVelocity() {
     stat1 = ks1->solve(LHS1, uhat.p(), rhs1.p(), *PC1, db->velocitySolve.tolerance, db->velocitySolve.maxIterations,
                        stream_);
     iterationCount1 = stat1.iterations;
+    itTotal1 += stat1.iterations;
+    restartsTotal += stat1.restarts;
+    bytesTotal1 += velocityBytes();
     if (!stat1.converged) {
         std::ostringstream m;
         m << "ERROR: Solve for uhat failed at time step " << timeStep << "\nIterations   : " << iterationCount1
@@ -213,6 +216,10 @@ void NavierStokesSolver::solvePoisson() {
     stat2 = ks2->solve(LHS2, pressure.p(), rhs2.p(), *PC2, db->PoissonSolve.tolerance, db->PoissonSolve.maxIterations,
                        stream_);
     iterationCount2 = stat2.iterations;
+    itTotal2 += stat2.iterations;
+    restartsTotal += stat2.restarts;
+    bytesTotal2 += poissonBytes();
+    solvesTotal2 += 1;
     if (!stat2.converged) {
         std::ostringstream m;
         m << "ERROR: Solve for pressure failed at time step " << timeStep << "\nIterations   : " << iterationCount2
@@ -326,6 +333,12 @@ void NavierStokesSolver::info(double *out, int n) const {
     v[18] = ny;
     v[19] = stat1.tolerance;
     v[20] = stat2.tolerance;
+    v[26] = itTotal1;
+    v[27] = itTotal2;
+    v[28] = bytesTotal1;
+    v[29] = bytesTotal2;
+    v[30] = solvesTotal2;
+    v[31] = restartsTotal;
     std::memcpy(out, v.data(), sizeof(double) * std::min(n, 32));
 }
 
@@ -378,10 +391,13 @@ double NavierStokesSolver::stepBytes() const {
 
 int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, double *bytes) {
     hipStream_t s = stream_;
-    DArray<double> t1(std::max(numUV, numP)), t2(std::max(numUV, numP));
+    DArray<double> t1(std::max(numUV, numP)), t2(std::max(numUV, numP)), part(2 * RED_GRID);
     double b = 0;
     auto once = [&]() {
         if (k == "spmv_lhs2") launch_spmv<double, double>(LHS2, pressure.p(), t1.p(), s);
+        else if (k == "spmv_dot_lhs2")
+            launch_spmv_dot(LHS2, pressure.p(), t1.p(), rhs2.p(), true, part.p(), s);
+        else if (k == "copy") launch_copy(t1.p(), pressure.p(), numP, s);
         else if (k == "spmv_lhs1") launch_spmv<double, double>(LHS1, uhat.p(), t1.p(), s);
         else if (k == "rhs1")
             launch_rhs1(u.p(), Nold.p(), t1.p(), t2.p(), nullptr, nullptr, grid(), bcs(), nu, dt, fadlunRhs(), s);
@@ -389,6 +405,8 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
         else throw Error("unknown kernel '" + k + "'", ERR_ARGUMENT);
     };
     if (k == "spmv_lhs2") b = csr_bytes(LHS2);
+    else if (k == "spmv_dot_lhs2") b = csr_bytes(LHS2) + 8.0 * numP + 16.0 * RED_GRID;  // + dot operand, partials
+    else if (k == "copy") b = 16.0 * numP;
     else if (k == "spmv_lhs1") b = csr_bytes(LHS1);
     else if (k == "rhs1") b = 32.0 * numUV + 8.0 * (2 * nx + 2 * ny) + 4 * 8.0 * (2 * ny + 2 * nx);
     else if (k == "vcycle") b = PC2 ? PC2->bytes_per_apply() : 0;

@@ -72,6 +72,8 @@ class NavierStokesSolver {
     int nx = 0, ny = 0, numU = 0, numV = 0, numUV = 0, numP = 0;
     double nu = 0, dt = 0;
     int timeStep = 0, iterationCount1 = 0, iterationCount2 = 0;
+    // running totals over all solves since initialise (for throughput accounting)
+    double itTotal1 = 0, itTotal2 = 0, bytesTotal1 = 0, bytesTotal2 = 0, solvesTotal2 = 0, restartsTotal = 0;
     SolveStat stat1, stat2;
 
     DArray<double> gx, gdx, gy, gdy, gxu, gyv;

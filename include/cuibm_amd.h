@@ -131,10 +131,16 @@ int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms);
 
 /* time `reps` back-to-back launches of one kernel of the step on its live inputs and
  * return the mean launch time in *ms and the algorithmic bytes per launch in *bytes.
- * kernel: "spmv_lhs2" (the Poisson operator product y = LHS2 x), "spmv_lhs1",
- * "rhs1" (fused intermediate-velocity right-hand side), "vcycle" (one Poisson
- * preconditioner application) */
+ * kernel: "spmv_lhs2" (the Poisson operator product y = LHS2 x), "spmv_dot_lhs2"
+ * (the same product fused with the two BiCGSTAB dot partials, as launched inside the
+ * Poisson solve), "spmv_lhs1", "rhs1" (fused intermediate-velocity right-hand side),
+ * "vcycle" (one Poisson preconditioner application), "copy" (a numP-double device
+ * copy, the calibration kernel for hardware byte counters) */
 int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms, double *bytes);
+
+/* block until all work on the current device has finished (hipDeviceSynchronize);
+ * the timing harness brackets its timed region with it. No reference counterpart. */
+int cuibm_device_synchronize(void);
 
 /* ---- linear algebra (replaces cusp::krylov::bicgstab / cusp::precond) ------------- */
 

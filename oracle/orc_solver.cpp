@@ -426,11 +426,12 @@ int orc_amg_vcycle(void *A, const double *b, double *x) {
 }
 
 // Galerkin product of the first level: writes A_1 = R A P as dense (n1 x n1); returns n1
-int orc_amg_coarse_dense(void *A, double *out, int cap) {
+int orc_amg_coarse_dense(void *A, int level, double *out, int cap) {
     orc::Amg G;
     G.setup(*(orc::Csr *)A);
-    if (G.lv.size() < 2) return -1;
-    const orc::Csr &Ac = G.lv[1].A;
+    if (level < 0) level = (int)G.lv.size() - 1;
+    if (level >= (int)G.lv.size()) return -1;
+    const orc::Csr &Ac = G.lv[level].A;
     if ((long)Ac.n * Ac.n > cap) return Ac.n;
     std::memset(out, 0, sizeof(double) * (size_t)Ac.n * Ac.n);
     for (int i = 0; i < Ac.n; i++)

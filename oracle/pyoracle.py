@@ -76,7 +76,7 @@ def lib():
         L.orc_amg_vcycle.restype = C.c_int
         L.orc_amg_vcycle.argtypes = [C.c_void_p, _DP, _DP]
         L.orc_amg_coarse_dense.restype = C.c_int
-        L.orc_amg_coarse_dense.argtypes = [C.c_void_p, _DP, C.c_int]
+        L.orc_amg_coarse_dense.argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
         L.orc_amg_prolongator_dense.restype = C.c_int
         L.orc_amg_prolongator_dense.argtypes = [C.c_void_p, _DP, C.c_int]
         _lib = L
@@ -282,9 +282,10 @@ class OrcCsr:
         lib().orc_amg_vcycle(self.h, dptr(b), dptr(x))
         return x
 
-    def coarse_dense(self, cap=4_000_000):
+    def coarse_dense(self, level=1, cap=4_000_000):
+        """dense Galerkin operator of hierarchy level `level` (-1: coarsest)"""
         out = np.zeros(cap)
-        n1 = lib().orc_amg_coarse_dense(self.h, dptr(out), cap)
+        n1 = lib().orc_amg_coarse_dense(self.h, level, dptr(out), cap)
         return out[: n1 * n1].reshape(n1, n1)
 
     def prolongator_dense(self, n1, cap=4_000_000):
