@@ -34,61 +34,79 @@ __global__ void k_jacobi_first(int n, const V *__restrict__ dinv, const double *
         x[i] = omega * (double)dinv[i] * b[i];
 }
 
-// xn = x + omega D^-1 (b - A x)
-template <class V>
-__global__ void __launch_bounds__(BS) k_jacobi(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                                               const V *__restrict__ v, const V *__restrict__ dinv,
-                                               const double *__restrict__ b, const double *__restrict__ x,
-                                               double *__restrict__ xn, double omega) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+// One CSR row product per group of LPR lanes (LPR = 1: a thread per row), with the
+// per-operator epilogue fused:
+//   OP_SPMV    y = A x
+//   OP_RESID   y = b - A x
+//   OP_JACOBI  y = x + omega D^-1 (b - A x)        (x is the operand of A)
+//   OP_PROLONG y += A x                            (A = P, x the coarse correction)
+// The lanes of a group add in a fixed shuffle tree, so results depend only on the
+// matrix and LPR, never on scheduling.
+enum { OP_SPMV = 0, OP_RESID, OP_JACOBI, OP_PROLONG };
+
+template <class V, int LPR, int OP>
+__global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                            const V *__restrict__ v, const double *__restrict__ x,
+                                            const double *__restrict__ b, const V *__restrict__ dinv,
+                                            double *__restrict__ y, double omega) {
+    const int lane = threadIdx.x & (LPR - 1);
+    const int per_block = BS / LPR;
+    for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         double s = 0;
-        for (int k = rp[i]; k < rp[i + 1]; k++) s += (double)v[k] * x[ci[k]];
-        xn[i] = x[i] + omega * (double)dinv[i] * (b[i] - s);
+        const int e = rp[i + 1];
+        for (int k = rp[i] + lane; k < e; k += LPR) s += (double)v[k] * x[ci[k]];
+        if (LPR > 1)
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
+        if (lane == 0) {
+            if (OP == OP_SPMV) y[i] = s;
+            else if (OP == OP_RESID) y[i] = b[i] - s;
+            else if (OP == OP_JACOBI) y[i] = x[i] + omega * (double)dinv[i] * (b[i] - s);
+            else y[i] += s;
+        }
     }
 }
 
-// r = b - A x
-template <class V>
-__global__ void __launch_bounds__(BS) k_resid(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                                              const V *__restrict__ v, const double *__restrict__ b,
-                                              const double *__restrict__ x, double *__restrict__ r) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double s = 0;
-        for (int k = rp[i]; k < rp[i + 1]; k++) s += (double)v[k] * x[ci[k]];
-        r[i] = b[i] - s;
-    }
+// lanes per row for a CSR operator, from its mean row length
+int lanes_for(const HCsr &A) {
+    const double avg = A.n > 0 ? (double)A.ci.size() / A.n : 0.0;
+    if (avg < 6) return 1;
+    if (avg < 12) return 4;
+    if (avg < 24) return 8;
+    if (avg < 48) return 16;
+    return 32;
 }
 
-// x += P xc
-template <class V>
-__global__ void __launch_bounds__(BS) k_prolong_add(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                                                    const V *__restrict__ v, const double *__restrict__ xc,
-                                                    double *__restrict__ x) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double s = 0;
-        for (int k = rp[i]; k < rp[i + 1]; k++) s += (double)v[k] * xc[ci[k]];
-        x[i] += s;
+template <class V, int OP>
+void launch_csr_op(int lpr, const DCsr<V> &A, const double *x, const double *b, const V *dinv, double *y, double omega,
+                   hipStream_t s) {
+    const int n = A.n;
+    if (n <= 0) return;
+    const long blocks = ((long)n * lpr + BS - 1) / BS;
+    const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
+    switch (lpr) {
+        case 1: hipLaunchKernelGGL((k_csr<V, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
+        case 4: hipLaunchKernelGGL((k_csr<V, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
+        case 8: hipLaunchKernelGGL((k_csr<V, 8, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
+        case 16: hipLaunchKernelGGL((k_csr<V, 16, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
+        default: hipLaunchKernelGGL((k_csr<V, 32, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
     }
+    CUIBM_LAUNCH_CHECK();
 }
 
-// x = Ainv b on the coarsest level (one block; b staged through LDS in tiles)
+// x = Ainv b on the coarsest level: one wavefront per row, lanes across the row
+// (row-major Ainv, coalesced), fixed shuffle-tree sum
 template <class V>
 __global__ void __launch_bounds__(256) k_dense(int n, const V *__restrict__ Ainv, const double *__restrict__ b,
                                                double *__restrict__ x) {
-    __shared__ double sb[256];
-    for (int i0 = 0; i0 < n; i0 += blockDim.x) {
-        const int i = i0 + threadIdx.x;
-        double s = 0;
-        for (int j0 = 0; j0 < n; j0 += 256) {
-            __syncthreads();
-            if (j0 + (int)threadIdx.x < n) sb[threadIdx.x] = b[j0 + threadIdx.x];
-            __syncthreads();
-            const int je = min(256, n - j0);
-            if (i < n)
-                for (int j = 0; j < je; j++) s += (double)Ainv[(size_t)i * n + j0 + j] * sb[j];
-        }
-        if (i < n) x[i] = s;
-    }
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;  // uniform per wavefront
+    double s = 0;
+    for (int j = lane; j < n; j += 64) s += (double)Ainv[(size_t)row * n + j] * b[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) x[row] = s;
 }
 
 }  // namespace
@@ -96,6 +114,7 @@ __global__ void __launch_bounds__(256) k_dense(int n, const V *__restrict__ Ainv
 template <class V>
 struct AmgLevelDev {
     int n = 0;
+    int lanesA = 1, lanesP = 1, lanesR = 1;  // lanes per row of each operator
     DCsr<V> A, P, R;
     DArray<V> dinv;
     DArray<double> b, x, xt, r;
@@ -114,9 +133,12 @@ struct AmgDev {
             AmgLevelDev<V> &L = lv[l];
             L.n = h[l].A.n;
             upload_csr(h[l].A, L.A, s);
+            L.lanesA = lanes_for(h[l].A);
             if (l + 1 < h.size()) {
                 upload_csr(h[l].P, L.P, s);
                 upload_csr(h[l].R, L.R, s);
+                L.lanesP = lanes_for(h[l].P);
+                L.lanesR = lanes_for(h[l].R);
             }
             std::vector<V> d(L.n, V(0));
             for (int i = 0; i < L.n; i++) {
@@ -147,7 +169,7 @@ struct AmgDev {
         const int n = L.n;
         const double w = prm.omega;
         if (l + 1 == lv.size()) {
-            hipLaunchKernelGGL(k_dense<V>, dim3(1), dim3(256), 0, s, n, coarse_inv.p(), b, x);
+            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), 0, s, n, coarse_inv.p(), b, x);
             CUIBM_LAUNCH_CHECK();
             return;
         }
@@ -159,23 +181,16 @@ struct AmgDev {
         hipLaunchKernelGGL(k_jacobi_first<V>, dim3(g), dim3(BS), 0, s, n, L.dinv.p(), b, cur, w);
         CUIBM_LAUNCH_CHECK();
         for (int k = 1; k < prm.npre; k++) {
-            hipLaunchKernelGGL(k_jacobi<V>, dim3(g), dim3(BS), 0, s, n, L.A.rp.p(), L.A.ci.p(), L.A.v.p(), L.dinv.p(),
-                               b, cur, oth, w);
-            CUIBM_LAUNCH_CHECK();
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s);
             std::swap(cur, oth);
         }
-        hipLaunchKernelGGL(k_resid<V>, dim3(g), dim3(BS), 0, s, n, L.A.rp.p(), L.A.ci.p(), L.A.v.p(), b, cur, L.r.p());
-        CUIBM_LAUNCH_CHECK();
+        launch_csr_op<V, OP_RESID>(L.lanesA, L.A, cur, b, nullptr, L.r.p(), 0.0, s);
         AmgLevelDev<V> &C = lv[l + 1];
-        launch_spmv<V, double>(L.R, L.r.p(), C.b.p(), s);
+        launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
         vcycle(l + 1, C.b.p(), C.x.p(), s);
-        hipLaunchKernelGGL(k_prolong_add<V>, dim3(g), dim3(BS), 0, s, n, L.P.rp.p(), L.P.ci.p(), L.P.v.p(), C.x.p(),
-                           cur);
-        CUIBM_LAUNCH_CHECK();
+        launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, cur, 0.0, s);
         for (int k = 0; k < prm.npost; k++) {
-            hipLaunchKernelGGL(k_jacobi<V>, dim3(g), dim3(BS), 0, s, n, L.A.rp.p(), L.A.ci.p(), L.A.v.p(), L.dinv.p(),
-                               b, cur, oth, w);
-            CUIBM_LAUNCH_CHECK();
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s);
             std::swap(cur, oth);
         }
     }
@@ -209,6 +224,23 @@ AmgPrecond::~AmgPrecond() = default;
 void AmgPrecond::apply(const double *r, double *z, hipStream_t s) {
     if (prm_.precision == 32) im_->d32.vcycle(0, r, z, s);
     else im_->d64.vcycle(0, r, z, s);
+}
+
+void AmgPrecond::level0_sweep(const double *b, const double *x, double *xn, hipStream_t s) {
+    if (prm_.precision == 32) {
+        auto &L = im_->d32.lv[0];
+        launch_csr_op<float, OP_JACOBI>(L.lanesA, L.A, x, b, L.dinv.p(), xn, prm_.omega, s);
+    } else {
+        auto &L = im_->d64.lv[0];
+        launch_csr_op<double, OP_JACOBI>(L.lanesA, L.A, x, b, L.dinv.p(), xn, prm_.omega, s);
+    }
+}
+
+double AmgPrecond::level0_sweep_bytes() const {
+    const double sv = prm_.precision == 32 ? 4.0 : 8.0;
+    const auto &A = im_->host[0].A;
+    const double n = A.n, nnz = (double)A.ci.size();
+    return nnz * (sv + 4) + 4 * (n + 1) + n * (8 + 8 + 8 + sv);  // A, rowptr, x, b, xn, dinv
 }
 
 AmgHierarchyInfo AmgPrecond::info() const {

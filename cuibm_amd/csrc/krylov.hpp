@@ -92,6 +92,10 @@ class AmgPrecond : public Preconditioner {
     std::string name() const override { return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)" : "SMOOTHED_AGGREGATION"; }
     double bytes_per_apply() const override;
     AmgHierarchyInfo info() const;
+    // one weighted-Jacobi sweep xn = x + omega D^-1 (b - A x) on the finest level, as
+    // run inside the V-cycle (for timing), and its algorithmic bytes
+    void level0_sweep(const double *b, const double *x, double *xn, hipStream_t s);
+    double level0_sweep_bytes() const;
 
   private:
     struct Impl;

@@ -362,6 +362,12 @@ std::string NavierStokesSolver::preconditionerName(int which) const {
     return p ? p->name() : "";
 }
 
+AmgPrecond *NavierStokesSolver::amg2() const {
+    auto *a = dynamic_cast<AmgPrecond *>(PC2.get());
+    if (!a) throw Error("the Poisson preconditioner is not SMOOTHED_AGGREGATION", ERR_ARGUMENT);
+    return a;
+}
+
 AmgHierarchyInfo NavierStokesSolver::amgInfo() const {
     if (auto *a = dynamic_cast<const AmgPrecond *>(PC2.get())) return a->info();
     return AmgHierarchyInfo();
@@ -392,24 +398,35 @@ double NavierStokesSolver::stepBytes() const {
 int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, double *bytes) {
     hipStream_t s = stream_;
     DArray<double> t1(std::max(numUV, numP)), t2(std::max(numUV, numP)), part(2 * RED_GRID);
+    // calibration copy: 64 MiB per array, larger than the L2 of every XCD together, so
+    // that back-to-back launches cannot serve reads from L2
+    const long kCopyN = 8l << 20;
+    DArray<double> c1, c2;
+    if (k == "copy") {
+        c1.alloc(kCopyN);
+        c2.alloc(kCopyN);
+        launch_fill(c1.p(), 1.0, kCopyN, s);
+    }
     double b = 0;
     auto once = [&]() {
         if (k == "spmv_lhs2") launch_spmv<double, double>(LHS2, pressure.p(), t1.p(), s);
         else if (k == "spmv_dot_lhs2")
             launch_spmv_dot(LHS2, pressure.p(), t1.p(), rhs2.p(), true, part.p(), s);
-        else if (k == "copy") launch_copy(t1.p(), pressure.p(), numP, s);
+        else if (k == "copy") launch_copy(c2.p(), c1.p(), kCopyN, s);
         else if (k == "spmv_lhs1") launch_spmv<double, double>(LHS1, uhat.p(), t1.p(), s);
         else if (k == "rhs1")
             launch_rhs1(u.p(), Nold.p(), t1.p(), t2.p(), nullptr, nullptr, grid(), bcs(), nu, dt, fadlunRhs(), s);
         else if (k == "vcycle") PC2->apply(rhs2.p(), t1.p(), s);
+        else if (k == "jacobi0") amg2()->level0_sweep(rhs2.p(), pressure.p(), t1.p(), s);
         else throw Error("unknown kernel '" + k + "'", ERR_ARGUMENT);
     };
     if (k == "spmv_lhs2") b = csr_bytes(LHS2);
     else if (k == "spmv_dot_lhs2") b = csr_bytes(LHS2) + 8.0 * numP + 16.0 * RED_GRID;  // + dot operand, partials
-    else if (k == "copy") b = 16.0 * numP;
+    else if (k == "copy") b = 16.0 * kCopyN;
     else if (k == "spmv_lhs1") b = csr_bytes(LHS1);
     else if (k == "rhs1") b = 32.0 * numUV + 8.0 * (2 * nx + 2 * ny) + 4 * 8.0 * (2 * ny + 2 * nx);
     else if (k == "vcycle") b = PC2 ? PC2->bytes_per_apply() : 0;
+    else if (k == "jacobi0") b = amg2()->level0_sweep_bytes();
     once();  // warm-up
     hipEvent_t a, e;
     CUIBM_HIP(hipEventCreate(&a));

@@ -62,6 +62,7 @@ class NavierStokesSolver {
     std::string preconditionerName(int which) const;
     // mean time of `reps` launches of one kernel on the live state (HIP events)
     int timeKernel(const std::string &kernel, int reps, double *ms, double *bytes);
+    AmgPrecond *amg2() const;  // the Poisson AMG preconditioner (throws if another)
     AmgHierarchyInfo amgInfo() const;
 
   protected:

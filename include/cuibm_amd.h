@@ -117,7 +117,10 @@ long cuibm_set_field(cuibm_solver *s, const char *name, const void *host, long n
 /* scalars: out[0] timeStep, [1..2] iterationCount1/2, [3..4] final residual norms,
  * [5..6] forceX/forceY, [7..9] FxX/FxY/FxU, [10..13] startI/startJ/numCellsX/Y,
  * [14..15] midX/midY, [16] AMG levels, [17..18] nx/ny, [19..20] solver tolerances,
- * [21..25] ghost/hybrid velocity, ghost/hybrid pressure and solid node counts */
+ * [21..25] ghost/hybrid velocity, ghost/hybrid pressure and solid node counts,
+ * running totals since initialise: [26..27] velocity/Poisson iterations,
+ * [28..29] velocity/Poisson solve bytes, [30] Poisson solves, [31] BiCGSTAB
+ * breakdown restarts */
 int cuibm_info(cuibm_solver *s, double *out, int n);
 
 /* algorithmic HBM bytes of the last step: which = 0 whole step, 1 velocity solve,
@@ -134,8 +137,9 @@ int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms);
  * kernel: "spmv_lhs2" (the Poisson operator product y = LHS2 x), "spmv_dot_lhs2"
  * (the same product fused with the two BiCGSTAB dot partials, as launched inside the
  * Poisson solve), "spmv_lhs1", "rhs1" (fused intermediate-velocity right-hand side),
- * "vcycle" (one Poisson preconditioner application), "copy" (a numP-double device
- * copy, the calibration kernel for hardware byte counters) */
+ * "vcycle" (one Poisson preconditioner application), "jacobi0" (one weighted-Jacobi
+ * sweep on the finest AMG level, as inside the V-cycle), "copy" (a copy of 8 Mi doubles,
+ * the calibration kernel for hardware byte counters) */
 int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms, double *bytes);
 
 /* block until all work on the current device has finished (hipDeviceSynchronize);
@@ -151,7 +155,8 @@ void cuibm_csr_destroy(cuibm_csr *A);
 int cuibm_csr_spmv(cuibm_csr *A, const double *x, double *y);
 /* cusp::krylov::bicgstab(A, x, b, cusp::monitor(b, maxit, rtol), M): x (host) is the
  * warm start and the result; precond = 0 NONE, 1 DIAGONAL, 2 SMOOTHED_AGGREGATION
- * (preconditioner.h:23-29); amg_precision = 64 or 32. Returns 0 when converged,
+ * (preconditioner.h:23-29); amg_precision = 64, or 32 for a hierarchy whose matrix
+ * values are stored in FP32 (vectors and arithmetic stay FP64). Returns 0 when converged,
  * CUIBM_ERR_NOT_CONVERGED otherwise, with *iters and *resid set either way. */
 int cuibm_bicgstab(cuibm_csr *A, double *x, const double *b, int precond, int amg_precision, double rtol, int maxit,
                    int *iters, double *resid);
