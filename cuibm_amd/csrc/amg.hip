@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "krylov.hpp"
+#include "stencil.hpp"
 
 namespace cuibm {
 
@@ -126,13 +127,16 @@ struct AmgDev {
     DArray<V> coarse_inv;
     int nc = 0;
     AmgParams prm;
+    bool use_fine = false;  // finest level in stencil form (fused kernels)
+    P5Dev fine{};
 
     void build(const std::vector<HostAmgLevel> &h, hipStream_t s) {
         lv.resize(h.size());
         for (size_t l = 0; l < h.size(); l++) {
             AmgLevelDev<V> &L = lv[l];
             L.n = h[l].A.n;
-            upload_csr(h[l].A, L.A, s);
+            const bool stencil_level = (l == 0 && use_fine);
+            if (!stencil_level) upload_csr(h[l].A, L.A, s);
             L.lanesA = lanes_for(h[l].A);
             if (l + 1 < h.size()) {
                 upload_csr(h[l].P, L.P, s);
@@ -140,14 +144,16 @@ struct AmgDev {
                 L.lanesP = lanes_for(h[l].P);
                 L.lanesR = lanes_for(h[l].R);
             }
-            std::vector<V> d(L.n, V(0));
-            for (int i = 0; i < L.n; i++) {
-                double a = 0;
-                for (int k = h[l].A.rp[i]; k < h[l].A.rp[i + 1]; k++)
-                    if (h[l].A.ci[k] == i) a += h[l].A.v[k];
-                d[i] = (V)(1.0 / a);
+            if (!stencil_level) {
+                std::vector<V> d(L.n, V(0));
+                for (int i = 0; i < L.n; i++) {
+                    double a = 0;
+                    for (int k = h[l].A.rp[i]; k < h[l].A.rp[i + 1]; k++)
+                        if (h[l].A.ci[k] == i) a += h[l].A.v[k];
+                    d[i] = (V)(1.0 / a);
+                }
+                L.dinv.upload(d, s);
             }
-            L.dinv.upload(d, s);
             if (l > 0) {  // level 0 works on the caller's vectors
                 L.b.alloc(L.n);
                 L.x.alloc(L.n);
@@ -171,6 +177,16 @@ struct AmgDev {
         if (l + 1 == lv.size()) {
             hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), 0, s, n, coarse_inv.p(), b, x);
             CUIBM_LAUNCH_CHECK();
+            return;
+        }
+        if (l == 0 && use_fine && prm.npre == 2 && prm.npost == 2) {
+            // fused finest level: x2 -> xt and r -> r, restriction, coarse cycle,
+            // prolongation and both post-sweeps straight into x
+            launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s);
+            AmgLevelDev<V> &C = lv[1];
+            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
+            vcycle(1, C.b.p(), C.x.p(), s);
+            launch_p5_post<V>(fine, w, L.P, C.x.p(), L.xt.p(), b, x, s);
             return;
         }
         const int g = grid_for(n);
@@ -202,13 +218,23 @@ struct AmgPrecond::Impl {
     std::vector<HostAmgLevel> host;
 };
 
-AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s) : im_(new Impl), prm_(p) {
+AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine)
+    : im_(new Impl), prm_(p) {
     im_->host = amg_build(A, p.coarse_max, p.max_levels);
+    fine_ = fine != nullptr && im_->host.size() > 1 && p.npre == 2 && p.npost == 2;
+    if (fine_) {
+        if (fine->nx * fine->ny != A.n) throw Error("stencil and matrix sizes differ", ERR_ARGUMENT);
+        fine_dev_ = *fine;
+    }
     if (p.precision == 32) {
         im_->d32.prm = p;
+        im_->d32.use_fine = fine_;
+        if (fine_) im_->d32.fine = *fine;
         im_->d32.build(im_->host, s);
     } else {
         im_->d64.prm = p;
+        im_->d64.use_fine = fine_;
+        if (fine_) im_->d64.fine = *fine;
         im_->d64.build(im_->host, s);
     }
     // keep only the sizes of the host hierarchy
@@ -227,7 +253,9 @@ void AmgPrecond::apply(const double *r, double *z, hipStream_t s) {
 }
 
 void AmgPrecond::level0_sweep(const double *b, const double *x, double *xn, hipStream_t s) {
-    if (prm_.precision == 32) {
+    if (fine_) {
+        launch_p5_sweep(fine_dev_, prm_.omega, x, b, xn, s);
+    } else if (prm_.precision == 32) {
         auto &L = im_->d32.lv[0];
         launch_csr_op<float, OP_JACOBI>(L.lanesA, L.A, x, b, L.dinv.p(), xn, prm_.omega, s);
     } else {
@@ -240,6 +268,7 @@ double AmgPrecond::level0_sweep_bytes() const {
     const double sv = prm_.precision == 32 ? 4.0 : 8.0;
     const auto &A = im_->host[0].A;
     const double n = A.n, nnz = (double)A.ci.size();
+    if (fine_) return n * (8 + 8 + 8) + 32.0 * (fine_dev_.nx + fine_dev_.ny);  // x, b, xn, coefficient arrays
     return nnz * (sv + 4) + 4 * (n + 1) + n * (8 + 8 + 8 + sv);  // A, rowptr, x, b, xn, dinv
 }
 
@@ -264,6 +293,13 @@ double AmgPrecond::bytes_per_apply() const {
     for (size_t l = 0; l + 1 < H.size(); l++) {
         const double n = H[l].A.n, nnz = (double)H[l].A.ci.size();
         const double nc = H[l + 1].A.n, nnzP = (double)H[l].P.ci.size();
+        if (l == 0 && fine_) {
+            const double coef = 32.0 * (fine_dev_.nx + fine_dev_.ny);
+            bytes += 3 * n * 8 + coef;                                   // pre: b in; x2, r out
+            bytes += spmv(nc, n, nnzP);                                  // restriction
+            bytes += nnzP * (sv + 4) + 4 * (n + 1) + nc * 8 + 3 * n * 8 + coef;  // post: P, xc, x2, b in; x out
+            continue;
+        }
         bytes += n * sv + 2 * n * 8;                                   // first sweep: dinv, b, x
         const int sweeps = prm_.npre - 1 + prm_.npost;
         bytes += sweeps * (spmv(n, n, nnz) + n * sv + n * 8);          // + dinv, b

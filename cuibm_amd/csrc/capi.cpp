@@ -341,7 +341,7 @@ int cuibm_bicgstab(cuibm_csr *A, double *x, const double *b, int precond, int am
         else throw Error("unsupported preconditioner", ERR_UNSUPPORTED);
         if (!M) M = own.get();
         Bicgstab ks(n);
-        const SolveStat st = ks.solve(A->A, dx.p(), db.p(), *M, rtol, maxit, A->stream);
+        const SolveStat st = ks.solve(CsrOp(A->A), dx.p(), db.p(), *M, rtol, maxit, A->stream);
         dx.download(x, n, A->stream);
         if (iters) *iters = st.iterations;
         if (resid) *resid = st.residual;

@@ -38,9 +38,9 @@ Bicgstab::~Bicgstab() {
     if (host_sc_) (void)hipHostFree(host_sc_);
 }
 
-SolveStat Bicgstab::solve(const DCsr<double> &A, double *x, const double *b, Preconditioner &M, double rtol,
+SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditioner &M, double rtol,
                           int maxit, hipStream_t s) {
-    if (A.n != n_) throw Error("BiCGSTAB workspace size mismatch", ERR_ARGUMENT);
+    if (A.n() != n_) throw Error("BiCGSTAB workspace size mismatch", ERR_ARGUMENT);
     const int n = n_;
     double *sc = sc_.p(), *part = part_.p();
     auto read = [&](int slot) {
@@ -54,7 +54,7 @@ SolveStat Bicgstab::solve(const DCsr<double> &A, double *x, const double *b, Pre
         CUIBM_HIP(hipStreamSynchronize(s));
     };
     auto start = [&]() {  // r = b - A x, rs = p = r
-        launch_bicg_init(A, x, b, r_.p(), p_.p(), rs_.p(), part, s);
+        A.init(x, b, r_.p(), p_.p(), rs_.p(), part, s);
         launch_bicg_finish(part, FIN_INIT, sc, s);
         fetch();
     };
@@ -65,7 +65,7 @@ SolveStat Bicgstab::solve(const DCsr<double> &A, double *x, const double *b, Pre
     st.tolerance = tol;
     while (!(rn <= tol) && it < maxit) {
         M.apply(p_.p(), Mp_.p(), s);
-        launch_spmv_dot(A, Mp_.p(), AMp_.p(), rs_.p(), false, part, s);
+        A.spmv_dot(Mp_.p(), AMp_.p(), rs_.p(), false, part, s);
         launch_bicg_finish(part, FIN_ALPHA, sc, s);
         launch_bicg_s(n, r_.p(), AMp_.p(), s_.p(), sc, part, s);
         launch_bicg_finish(part, FIN_SS, sc, s);
@@ -83,7 +83,7 @@ SolveStat Bicgstab::solve(const DCsr<double> &A, double *x, const double *b, Pre
             break;
         }
         M.apply(s_.p(), Ms_.p(), s);
-        launch_spmv_dot(A, Ms_.p(), AMs_.p(), s_.p(), true, part, s);
+        A.spmv_dot(Ms_.p(), AMs_.p(), s_.p(), true, part, s);
         launch_bicg_finish(part, FIN_OMEGA, sc, s);
         launch_bicg_xr(n, x, Mp_.p(), Ms_.p(), r_.p(), s_.p(), AMs_.p(), rs_.p(), sc, part, s);
         launch_bicg_finish(part, FIN_BETA, sc, s);

@@ -37,6 +37,39 @@ void launch_dot2(long n, const double *x, const double *y, double *part, hipStre
 void launch_jacobi_apply(int n, const double *dinv, const double *r, double *z, hipStream_t s);
 void launch_diag_inv(const DCsr<double> &A, double *dinv, hipStream_t s);
 
+// ---- linear operators (the A of BiCGSTAB)
+class LinOp {
+  public:
+    virtual ~LinOp() = default;
+    virtual int n() const = 0;
+    // y = A x (y may be null) with partials of (y . a) and, if want_yy, (y . y)
+    virtual void spmv_dot(const double *x, double *y, const double *a, bool want_yy, double *part,
+                          hipStream_t s) const = 0;
+    // r = b - A x, p = rs = r, partials of (r . r) and (b . b)
+    virtual void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
+                      hipStream_t s) const = 0;
+    // algorithmic bytes of one product y = A x
+    virtual double spmv_bytes() const = 0;
+};
+
+class CsrOp : public LinOp {
+  public:
+    explicit CsrOp(const DCsr<double> &A) : A_(A) {}
+    int n() const override { return A_.n; }
+    void spmv_dot(const double *x, double *y, const double *a, bool want_yy, double *part,
+                  hipStream_t s) const override {
+        launch_spmv_dot(A_, x, y, a, want_yy, part, s);
+    }
+    void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
+              hipStream_t s) const override {
+        launch_bicg_init(A_, x, b, r, p, rs, part, s);
+    }
+    double spmv_bytes() const override { return A_.nnz * 12.0 + 4.0 * (A_.n + 1) + 16.0 * A_.n; }
+
+  private:
+    const DCsr<double> &A_;
+};
+
 // ---- preconditioners
 class Preconditioner {
   public:
@@ -84,9 +117,19 @@ struct AmgHierarchyInfo {
     std::vector<int> n, nnz, nnzP;
 };
 
+// the pressure-Poisson operator as a 5-point stencil (see stencil.hpp)
+struct P5Dev {
+    int nx, ny;
+    int centre;                        // cell whose diagonal is doubled besides cell 0, or -1
+    const double *cE, *cW, *cN, *cS;   // coupling magnitudes; 0 where the neighbour is missing
+};
+
 class AmgPrecond : public Preconditioner {
   public:
-    AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s);
+    // `fine`, when given, is the stencil form of A: the finest level then runs the
+    // fused stencil kernels instead of CSR products with A (the hierarchy below is
+    // still built from the CSR A)
+    AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine = nullptr);
     ~AmgPrecond() override;
     void apply(const double *r, double *z, hipStream_t s) override;
     std::string name() const override { return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)" : "SMOOTHED_AGGREGATION"; }
@@ -101,6 +144,8 @@ class AmgPrecond : public Preconditioner {
     struct Impl;
     std::unique_ptr<Impl> im_;
     AmgParams prm_;
+    bool fine_ = false;
+    P5Dev fine_dev_{};
 };
 
 // host smoothed-aggregation set-up (amg_setup.cpp)
@@ -123,7 +168,7 @@ class Bicgstab {
     explicit Bicgstab(int n);
     ~Bicgstab();
     // cusp::krylov::bicgstab + cusp::monitor(b, maxit, rtol) semantics; x is the warm start
-    SolveStat solve(const DCsr<double> &A, double *x, const double *b, Preconditioner &M, double rtol, int maxit,
+    SolveStat solve(const LinOp &A, double *x, const double *b, Preconditioner &M, double rtol, int maxit,
                     hipStream_t s);
     int n() const { return n_; }
 

@@ -13,6 +13,7 @@
 #include <sstream>
 
 #include "solver.hpp"
+#include "stencil.hpp"
 
 namespace cuibm {
 
@@ -149,14 +150,15 @@ static HCsr download_csr(const DCsr<double> &A, hipStream_t s) {
     return h;
 }
 
-static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p, const DCsr<double> &A, hipStream_t s) {
+static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p, const DCsr<double> &A, hipStream_t s,
+                                                    const P5Dev *fine = nullptr) {
     switch (p.preconditioner) {
         case NONE: return std::unique_ptr<Preconditioner>(new IdentityPrecond(A.n));
         case DIAGONAL: return std::unique_ptr<Preconditioner>(new JacobiPrecond(A, s));
         case SMOOTHED_AGGREGATION: {
             AmgParams ap;
             ap.precision = p.amgPrecision;
-            return std::unique_ptr<Preconditioner>(new AmgPrecond(download_csr(A, s), ap, s));
+            return std::unique_ptr<Preconditioner>(new AmgPrecond(download_csr(A, s), ap, s, fine));
         }
         default: throw Error("preconditioner AINV is not supported", ERR_UNSUPPORTED);
     }
@@ -164,8 +166,17 @@ static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p,
 
 // newPrecon.cu:17-35 / preconditioner.h:84-111
 void NavierStokesSolver::buildPreconditioners() {
+    op1.reset(new CsrOp(LHS1));
+    P5Dev fine{};
+    if (stencilLHS2()) {  // LHS2 rows are the plain formula: use the stencil form
+        auto *p5 = new PoissonStencil(grid(), dt, centrePin(), stream_);
+        fine = p5->dev();
+        op2.reset(p5);
+    } else {
+        op2.reset(new CsrOp(LHS2));
+    }
     PC1 = make_precond(db->velocitySolve, LHS1, stream_);
-    PC2 = make_precond(db->PoissonSolve, LHS2, stream_);
+    PC2 = make_precond(db->PoissonSolve, LHS2, stream_, stencilLHS2() ? &fine : nullptr);
     ks1.reset(new Bicgstab(numUV));
     ks2.reset(new Bicgstab(numP));
     CUIBM_HIP(hipStreamSynchronize(stream_));
@@ -198,7 +209,7 @@ void NavierStokesSolver::velocityProjection() { projectPlain(); }
 
 // NavierStokesSolver.cu:221-271
 void NavierStokesSolver::solveIntermediateVelocity() {
-    stat1 = ks1->solve(LHS1, uhat.p(), rhs1.p(), *PC1, db->velocitySolve.tolerance, db->velocitySolve.maxIterations,
+    stat1 = ks1->solve(*op1, uhat.p(), rhs1.p(), *PC1, db->velocitySolve.tolerance, db->velocitySolve.maxIterations,
                        stream_);
     iterationCount1 = stat1.iterations;
     itTotal1 += stat1.iterations;
@@ -213,7 +224,7 @@ void NavierStokesSolver::solveIntermediateVelocity() {
 }
 
 void NavierStokesSolver::solvePoisson() {
-    stat2 = ks2->solve(LHS2, pressure.p(), rhs2.p(), *PC2, db->PoissonSolve.tolerance, db->PoissonSolve.maxIterations,
+    stat2 = ks2->solve(*op2, pressure.p(), rhs2.p(), *PC2, db->PoissonSolve.tolerance, db->PoissonSolve.maxIterations,
                        stream_);
     iterationCount2 = stat2.iterations;
     itTotal2 += stat2.iterations;
@@ -350,7 +361,11 @@ int NavierStokesSolver::phase(const std::string &n) {
     else if (n == "velocityProjection" || n == "project") velocityProjection();
     else if (n == "generateLHS1") generateLHS1();
     else if (n == "generateLHS2") generateLHS2();
-    else if (n == "rhs1_only") {  // the fused rhs1 sweep on the current N/Nold, without the swap
+    else if (n == "poisson_operator") {  // rhs2 = A p with the operator the Poisson solve uses
+        DArray<double> part(2 * RED_GRID);
+        op2->spmv_dot(pressure.p(), rhs2.p(), pressure.p(), false, part.p(), stream_);
+        CUIBM_HIP(hipStreamSynchronize(stream_));
+    } else if (n == "rhs1_only") {  // the fused rhs1 sweep on the current N/Nold, without the swap
         launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), L.p(), bc1.p(), grid(), bcs(), nu, dt, fadlunRhs(), stream_);
     } else return -1;
     CUIBM_HIP(hipStreamSynchronize(stream_));
@@ -375,8 +390,8 @@ AmgHierarchyInfo NavierStokesSolver::amgInfo() const {
 
 
 // algorithmic bytes of one BiCGSTAB solve as run (distinct arrays per kernel)
-static double krylov_bytes(const DCsr<double> &A, const Preconditioner &M, const SolveStat &st) {
-    const double n = A.n, spmv = csr_bytes(A);
+static double krylov_bytes(const LinOp &A, const Preconditioner &M, const SolveStat &st) {
+    const double n = A.n(), spmv = A.spmv_bytes();
     double b = spmv + 4 * 8 * n;  // init: r = b - Ax, writes r, p, rs
     const double full = 2 * (M.bytes_per_apply() + spmv + 8 * n)  // two preconditioner + spmv-dot passes
                         + 3 * 8 * n                                // s = r - alpha AMp
@@ -387,8 +402,8 @@ static double krylov_bytes(const DCsr<double> &A, const Preconditioner &M, const
     return b;
 }
 
-double NavierStokesSolver::poissonBytes() const { return PC2 ? krylov_bytes(LHS2, *PC2, stat2) : 0.0; }
-double NavierStokesSolver::velocityBytes() const { return PC1 ? krylov_bytes(LHS1, *PC1, stat1) : 0.0; }
+double NavierStokesSolver::poissonBytes() const { return PC2 ? krylov_bytes(*op2, *PC2, stat2) : 0.0; }
+double NavierStokesSolver::velocityBytes() const { return PC1 ? krylov_bytes(*op1, *PC1, stat1) : 0.0; }
 double NavierStokesSolver::stepBytes() const {
     // rhs1 sweep (u, Nold in; N, rhs1 out), rhs2 (uhat in, rhs2 out), projection (uhat, p, u in; u, uold out)
     const double stencils = 32.0 * numUV + 8.0 * (numUV + numP) + 8.0 * (3 * numUV + numP);
@@ -410,8 +425,7 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
     double b = 0;
     auto once = [&]() {
         if (k == "spmv_lhs2") launch_spmv<double, double>(LHS2, pressure.p(), t1.p(), s);
-        else if (k == "spmv_dot_lhs2")
-            launch_spmv_dot(LHS2, pressure.p(), t1.p(), rhs2.p(), true, part.p(), s);
+        else if (k == "spmv_dot_lhs2") op2->spmv_dot(pressure.p(), t1.p(), rhs2.p(), true, part.p(), s);
         else if (k == "copy") launch_copy(c2.p(), c1.p(), kCopyN, s);
         else if (k == "spmv_lhs1") launch_spmv<double, double>(LHS1, uhat.p(), t1.p(), s);
         else if (k == "rhs1")
@@ -421,7 +435,7 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
         else throw Error("unknown kernel '" + k + "'", ERR_ARGUMENT);
     };
     if (k == "spmv_lhs2") b = csr_bytes(LHS2);
-    else if (k == "spmv_dot_lhs2") b = csr_bytes(LHS2) + 8.0 * numP + 16.0 * RED_GRID;  // + dot operand, partials
+    else if (k == "spmv_dot_lhs2") b = op2->spmv_bytes() + 8.0 * numP + 16.0 * RED_GRID;  // + dot operand, partials
     else if (k == "copy") b = 16.0 * kCopyN;
     else if (k == "spmv_lhs1") b = csr_bytes(LHS1);
     else if (k == "rhs1") b = 32.0 * numUV + 8.0 * (2 * nx + 2 * ny) + 4 * 8.0 * (2 * ny + 2 * nx);

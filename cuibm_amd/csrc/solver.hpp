@@ -62,6 +62,9 @@ class NavierStokesSolver {
     std::string preconditionerName(int which) const;
     // mean time of `reps` launches of one kernel on the live state (HIP events)
     int timeKernel(const std::string &kernel, int reps, double *ms, double *bytes);
+    // LHS2 rows follow the plain formula (no IBM row overrides): the Poisson solve
+    // can use the stencil form of the operator
+    virtual bool stencilLHS2() const { return true; }
     AmgPrecond *amg2() const;  // the Poisson AMG preconditioner (throws if another)
     AmgHierarchyInfo amgInfo() const;
 
@@ -80,6 +83,7 @@ class NavierStokesSolver {
     DArray<double> gx, gdx, gy, gdy, gxu, gyv;
     DArray<double> u, uhat, uold, pressure, N, Nold, L, bc1, rhs1, rhs2, bc[4];
     DCsr<double> LHS1, LHS2;
+    std::unique_ptr<LinOp> op1, op2;  // the velocity and Poisson operators of the two solves
     std::unique_ptr<Preconditioner> PC1, PC2;
     std::unique_ptr<Bicgstab> ks1, ks2;
     std::ofstream iterationsFile;
@@ -200,6 +204,7 @@ class fadlunModified : public luo_base {
     FadlunRhsDev fadlunRhs() const override;
     FadlunP2Dev fadlunP2() const override;
     FadlunLhsDev fadlunLhs() const override;
+    bool stencilLHS2() const override { return false; }  // cut-cell and ghost rows differ
 };
 
 // factory, the switch of src/cuIBM.cu:42-60

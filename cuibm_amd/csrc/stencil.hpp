@@ -1,0 +1,54 @@
+// The pressure-Poisson operator LHS2 of the plain and Luo solvers as a 5-point
+// stencil over four 1-D coefficient arrays, instead of a CSR matrix.
+//
+// LHS2 (NavierStokes/kernels/LHS2.cu:23-173, luoIBM/kernels/LHS2.cu:12-62) couples
+// cell (I,J) to its east neighbour with -dt/(dx_I (dx_I + dx_{I+1})/2), and likewise
+// west, north and south; the diagonal is the sum of the four magnitudes (walls drop
+// the missing neighbour), doubled at cell 0 and, for the plain solver, at the centre
+// cell. Every coefficient depends on I alone (east/west) or J alone (north/south),
+// so O(nx + ny) doubles describe the whole matrix. Products are evaluated in the CSR
+// row order S, W, C, E, N with the same expressions as the assembled matrix, so
+// y = A x is bit-identical to the CSR product.
+//
+// The V-cycle's finest level runs on this form with two fused LDS-tiled kernels
+// (pre-smoothing + residual, prolongation + post-smoothing) instead of seven CSR
+// launches; each block owns a 64 x 16 tile of cells and computes the intermediate
+// iterates on a halo, so the kernels read b (and x) once and write only their
+// results. Blocks are assigned tiles in XCD-contiguous bands (the hardware deals
+// workgroups to the 8 XCDs round-robin), so the halo rows a block needs sit in its
+// own XCD's L2.
+#pragma once
+#include "kernels.hpp"
+#include "krylov.hpp"
+
+namespace cuibm {
+
+class PoissonStencil : public LinOp {
+  public:
+    PoissonStencil(const GridDev &g, double dt, bool centre_pin, hipStream_t s);
+    int n() const override { return nx_ * ny_; }
+    void spmv_dot(const double *x, double *y, const double *a, bool want_yy, double *part,
+                  hipStream_t s) const override;
+    void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
+              hipStream_t s) const override;
+    double spmv_bytes() const override { return 16.0 * n() + 32.0 * (nx_ + ny_); }
+    P5Dev dev() const { return P5Dev{nx_, ny_, centre_, cE_.p(), cW_.p(), cN_.p(), cS_.p()}; }
+    // y = A x without dot partials (tests)
+    void apply(const double *x, double *y, hipStream_t s) const;
+
+  private:
+    int nx_, ny_, centre_;
+    DArray<double> cE_, cW_, cN_, cS_;
+};
+
+// finest-level V-cycle pieces (omega-weighted Jacobi, D = diag(A)):
+//   pre:  x1 = omega D^-1 b, x2 = x1 + omega D^-1 (b - A x1), r = b - A x2
+//   post: x = x2 + P xc, then two sweeps x <- x + omega D^-1 (b - A x)
+void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s);
+template <class V>
+void launch_p5_post(const P5Dev &A, double omega, const DCsr<V> &P, const double *xc, const double *x2,
+                    const double *b, double *x, hipStream_t s);
+// one sweep xn = x + omega D^-1 (b - A x) (timing of the finest-level smoother)
+void launch_p5_sweep(const P5Dev &A, double omega, const double *x, const double *b, double *xn, hipStream_t s);
+
+}  // namespace cuibm
