@@ -21,7 +21,7 @@ about 55 steps in this method (GPU and CPU restatement alike); the default windo
 (steps 6-45) lies before that.
 
 Prints one JSON line. ``roofline`` reports the dominant kernel of the step (the
-weighted-Jacobi sweep on the finest level of the Poisson AMG hierarchy, three per
+fused finest-level pre-smoothing + residual kernel of the Poisson V-cycle, one per
 V-cycle and two V-cycles per BiCGSTAB iteration) timed with HIP events over
 back-to-back launches on the live hierarchy; ``traffic`` is the
 HBM byte count per launch of that kernel from hardware counters, read from the
@@ -42,10 +42,10 @@ sys.path.insert(0, HERE)
 METRIC = ("time steps/s + HBM GB/s at 1024² (mixed vs FP64, same C_D accuracy); adjoint s/grad")
 CASE = "cylinderRe550_1024_luo"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_jacobi0_cylinder1024.md")
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_vc_pre_cylinder1024.md")
 # kernels timed beside the roofline kernel: the Poisson operator product with the
 # BiCGSTAB dots, one whole V-cycle, the fused intermediate-velocity right-hand side
-EXTRA_KERNELS = ["spmv_dot_lhs2", "vcycle", "rhs1"]
+EXTRA_KERNELS = ["vc_post", "spmv_dot_lhs2", "vcycle", "rhs1", "jacobi0"]
 
 
 def log(*a):
@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--kernel", default="jacobi0", help="roofline kernel (see cuibm_time_kernel)")
+    ap.add_argument("--kernel", default="vc_pre", help="roofline kernel (see cuibm_time_kernel)")
     ap.add_argument("--kernel-reps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-hierarchy comparison run")

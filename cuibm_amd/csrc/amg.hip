@@ -4,8 +4,8 @@
 // One application is one V-cycle from a zero initial guess: on every level but the
 // coarsest, npre weighted-Jacobi sweeps (omega = 0.8; the first is x = omega D^-1 b),
 // residual, restriction with R = P^T, recursion, prolongation-correction, npost
-// sweeps; the coarsest level (<= 256 unknowns) applies a dense inverse with one
-// block.
+// sweeps; the coarsest level (<= 2500 unknowns) applies a dense inverse, one
+// wavefront per row.
 //
 // Mixed precision stores only the matrix values (A_l, P_l, R_l, D_l^-1 and the
 // coarse inverse) as float; the level vectors and every product, sum and update stay
@@ -66,6 +66,48 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
             else y[i] += s;
         }
     }
+}
+
+// Both pre-smoothing sweeps of a CSR level in one pass: x2 = x1 + omega D^-1 (b - A x1)
+// with x1 = omega D^-1 b evaluated at the row's neighbours on the fly. (Fusing the
+// prolongation into the first post-sweep the same way was measured slower: every row
+// then gathers its neighbours' whole P rows.)
+template <class V, int LPR>
+__global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                             const V *__restrict__ v, const V *__restrict__ dinv,
+                                             const double *__restrict__ b, double *__restrict__ x2, double omega) {
+    const int lane = threadIdx.x & (LPR - 1);
+    const int per_block = BS / LPR;
+    for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+        double s = 0;
+        const int e = rp[i + 1];
+        for (int k = rp[i] + lane; k < e; k += LPR) {
+            const int j = ci[k];
+            s += (double)v[k] * (omega * (double)dinv[j] * b[j]);
+        }
+        if (LPR > 1)
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
+        if (lane == 0) {
+            const double wd = omega * (double)dinv[i];
+            x2[i] = wd * b[i] + wd * (b[i] - s);
+        }
+    }
+}
+
+template <class V>
+void launch_pre2(int lpr, const DCsr<V> &A, const V *dinv, const double *b, double *x2, double omega, hipStream_t s) {
+    const int n = A.n;
+    const long blocks = ((long)n * lpr + BS - 1) / BS;
+    const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
+    switch (lpr) {
+        case 1: hipLaunchKernelGGL((k_pre2<V, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
+        case 4: hipLaunchKernelGGL((k_pre2<V, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
+        case 8: hipLaunchKernelGGL((k_pre2<V, 8>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
+        case 16: hipLaunchKernelGGL((k_pre2<V, 16>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
+        default: hipLaunchKernelGGL((k_pre2<V, 32>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
+    }
+    CUIBM_LAUNCH_CHECK();
 }
 
 // lanes per row for a CSR operator, from its mean row length
@@ -162,7 +204,7 @@ struct AmgDev {
             L.r.alloc(L.n);
         }
         nc = h.back().A.n;
-        if (nc > 4096) throw Error("AMG coarsening stalled at " + std::to_string(nc) + " unknowns", ERR_GENERIC);
+        if (nc > 8192) throw Error("AMG coarsening stalled at " + std::to_string(nc) + " unknowns", ERR_GENERIC);
         const std::vector<double> inv = dense_inverse(h.back().A);
         std::vector<V> invt(inv.begin(), inv.end());
         coarse_inv.upload(invt, s);
@@ -180,13 +222,27 @@ struct AmgDev {
             return;
         }
         if (l == 0 && use_fine && prm.npre == 2 && prm.npost == 2) {
-            // fused finest level: x2 -> xt and r -> r, restriction, coarse cycle,
-            // prolongation and both post-sweeps straight into x
+            // stencil finest level: x2 -> xt and r -> r, restriction, coarse cycle,
+            // prolongation into xt, both post-sweeps straight into x
             launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s);
             AmgLevelDev<V> &C = lv[1];
             launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
             vcycle(1, C.b.p(), C.x.p(), s);
-            launch_p5_post<V>(fine, w, L.P, C.x.p(), L.xt.p(), b, x, s);
+            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s);
+            launch_p5_sweep2(fine, w, L.xt.p(), b, x, s);
+            return;
+        }
+        if (prm.npre == 2 && prm.npost == 2) {
+            // x2 -> xt, r -> r, restriction, coarse cycle, prolongation into xt,
+            // post-sweeps xt -> r -> x
+            AmgLevelDev<V> &C = lv[l + 1];
+            launch_pre2<V>(L.lanesA, L.A, L.dinv.p(), b, L.xt.p(), w, s);
+            launch_csr_op<V, OP_RESID>(L.lanesA, L.A, L.xt.p(), b, nullptr, L.r.p(), 0.0, s);
+            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
+            vcycle(l + 1, C.b.p(), C.x.p(), s);
+            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s);
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), b, L.dinv.p(), L.r.p(), w, s);
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.r.p(), b, L.dinv.p(), x, w, s);
             return;
         }
         const int g = grid_for(n);
@@ -272,6 +328,35 @@ double AmgPrecond::level0_sweep_bytes() const {
     return nnz * (sv + 4) + 4 * (n + 1) + n * (8 + 8 + 8 + sv);  // A, rowptr, x, b, xn, dinv
 }
 
+void AmgPrecond::level0_pre(const double *b, double *x2, double *r, hipStream_t s) {
+    if (!fine_) throw Error("the finest AMG level is not in stencil form", ERR_ARGUMENT);
+    launch_p5_pre(fine_dev_, prm_.omega, b, x2, r, s);
+}
+
+void AmgPrecond::level0_post(double *x2, const double *b, double *x, hipStream_t s) {
+    if (!fine_) throw Error("the finest AMG level is not in stencil form", ERR_ARGUMENT);
+    if (prm_.precision == 32) {
+        auto &L = im_->d32.lv[0];
+        launch_csr_op<float, OP_PROLONG>(L.lanesP, L.P, im_->d32.lv[1].x.p(), nullptr, nullptr, x2, 0.0, s);
+    } else {
+        auto &L = im_->d64.lv[0];
+        launch_csr_op<double, OP_PROLONG>(L.lanesP, L.P, im_->d64.lv[1].x.p(), nullptr, nullptr, x2, 0.0, s);
+    }
+    launch_p5_sweep2(fine_dev_, prm_.omega, x2, b, x, s);
+}
+
+double AmgPrecond::level0_pre_bytes() const {
+    return 4.0 * 8 * im_->host[0].A.n + 32.0 * (fine_dev_.nx + fine_dev_.ny);  // b, D^-1 in; x2, r out
+}
+
+double AmgPrecond::level0_post_bytes() const {
+    const double sv = prm_.precision == 32 ? 4.0 : 8.0;
+    const double n = im_->host[0].A.n, nc = im_->host[1].A.n, nnzP = (double)im_->host[0].P.ci.size();
+    const double prolong = nnzP * (sv + 4) + 4 * (n + 1) + nc * 8 + 2 * n * 8;            // P, xc; x2 read-modify-write
+    const double sweeps = 4.0 * 8 * n + 32.0 * (fine_dev_.nx + fine_dev_.ny);            // x, b, D^-1 in; x out
+    return prolong + sweeps;
+}
+
 AmgHierarchyInfo AmgPrecond::info() const {
     AmgHierarchyInfo I;
     I.levels = (int)im_->host.size();
@@ -294,10 +379,7 @@ double AmgPrecond::bytes_per_apply() const {
         const double n = H[l].A.n, nnz = (double)H[l].A.ci.size();
         const double nc = H[l + 1].A.n, nnzP = (double)H[l].P.ci.size();
         if (l == 0 && fine_) {
-            const double coef = 32.0 * (fine_dev_.nx + fine_dev_.ny);
-            bytes += 3 * n * 8 + coef;                                   // pre: b in; x2, r out
-            bytes += spmv(nc, n, nnzP);                                  // restriction
-            bytes += nnzP * (sv + 4) + 4 * (n + 1) + nc * 8 + 3 * n * 8 + coef;  // post: P, xc, x2, b in; x out
+            bytes += level0_pre_bytes() + spmv(nc, n, nnzP) + level0_post_bytes();  // pre, restriction, post
             continue;
         }
         bytes += n * sv + 2 * n * 8;                                   // first sweep: dinv, b, x

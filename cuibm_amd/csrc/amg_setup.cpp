@@ -10,7 +10,7 @@
 //   tentative prolongator: piecewise constant, each column scaled to unit norm;
 //   smoothed prolongator P = (I - 4/(3 rho) D^-1 A) T, rho = max_i sum_j |a_ij/a_ii|;
 //   restriction R = P^T, coarse operator A_c = R A P;
-//   coarsening stops at <= coarse_max unknowns, solved with a dense inverse.
+//   coarsening stops at <= coarse_max unknowns (2500), solved with a dense inverse.
 // Everything is sequential and in index order, so the hierarchy is deterministic.
 #include <algorithm>
 #include <cmath>
@@ -207,19 +207,18 @@ std::vector<double> dense_inverse(const HCsr &A) {
                 std::swap(M[(size_t)c * n + k], M[(size_t)piv * n + k]);
                 std::swap(X[(size_t)c * n + k], X[(size_t)piv * n + k]);
             }
+        // columns left of c are already eliminated in M (zero off the diagonal)
         const double inv = 1.0 / M[(size_t)c * n + c];
-        for (int k = 0; k < n; k++) {
-            M[(size_t)c * n + k] *= inv;
-            X[(size_t)c * n + k] *= inv;
-        }
+        for (int k = c; k < n; k++) M[(size_t)c * n + k] *= inv;
+        for (int k = 0; k < n; k++) X[(size_t)c * n + k] *= inv;
         for (int r = 0; r < n; r++) {
             if (r == c) continue;
             const double f = M[(size_t)r * n + c];
             if (f == 0.0) continue;
-            for (int k = 0; k < n; k++) {
-                M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
-                X[(size_t)r * n + k] -= f * X[(size_t)c * n + k];
-            }
+            double *mr = &M[(size_t)r * n], *xr = &X[(size_t)r * n];
+            const double *mc = &M[(size_t)c * n], *xc = &X[(size_t)c * n];
+            for (int k = c; k < n; k++) mr[k] -= f * mc[k];
+            for (int k = 0; k < n; k++) xr[k] -= f * xc[k];
         }
     }
     return X;

@@ -124,36 +124,98 @@ __global__ void __launch_bounds__(BS) k_p5_init(P5Dev A, const double *__restric
 
 __device__ __forceinline__ bool inside(const P5Dev &A, int I, int J) { return I >= 0 && I < A.nx && J >= 0 && J < A.ny; }
 
-// pre-smoothing (two sweeps from x = 0) and residual on one tile
+// D^-1 of the operator (the same 1.0 / diagonal a CSR sweep would use)
+__global__ void k_p5_dinv(P5Dev A, double *__restrict__ dinv) {
+    const int n = A.nx * A.ny;
+    for (int ip = blockIdx.x * blockDim.x + threadIdx.x; ip < n; ip += gridDim.x * blockDim.x)
+        dinv[ip] = 1.0 / p5_diag(A, ip % A.nx, ip / A.nx, ip);
+}
+
+// the coupling magnitudes of the columns and rows one tile (with a halo of 2) touches,
+// staged in LDS; li, lj below are halo-2 tile coordinates (I = I0 - 2 + li)
+struct TileCoef {
+    double e[W2], w[W2], n[H2], s[H2];
+};
+
+__device__ __forceinline__ void load_coef(const P5Dev &A, int I0, int J0, TileCoef &c) {
+    for (int k = threadIdx.x; k < W2; k += blockDim.x) {
+        const int I = I0 - 2 + k;
+        const bool in = I >= 0 && I < A.nx;
+        c.e[k] = in ? A.cE[I] : 0.0;
+        c.w[k] = in ? A.cW[I] : 0.0;
+    }
+    for (int k = threadIdx.x; k < H2; k += blockDim.x) {
+        const int J = J0 - 2 + k;
+        const bool in = J >= 0 && J < A.ny;
+        c.n[k] = in ? A.cN[J] : 0.0;
+        c.s[k] = in ? A.cS[J] : 0.0;
+    }
+}
+
+__device__ __forceinline__ double t_diag(const P5Dev &A, const TileCoef &c, int li, int lj, int I, int J, int ip) {
+    double t = 0;
+    if (I != A.nx - 1) t += c.e[li];
+    if (I != 0) t += c.w[li];
+    if (J != A.ny - 1) t += c.n[lj];
+    if (J != 0) t += c.s[lj];
+    if (ip == 0) t += t;
+    if (ip == A.centre) t += t;
+    return t;
+}
+
+template <class F>
+__device__ __forceinline__ double t_row(const P5Dev &A, const TileCoef &c, int li, int lj, int I, int J, double d, F x,
+                                        int sy) {
+    double s = 0;
+    if (J != 0) s += (-c.s[lj]) * x(-sy);
+    if (I != 0) s += (-c.w[li]) * x(-1);
+    s += d * x(0);
+    if (I != A.nx - 1) s += (-c.e[li]) * x(1);
+    if (J != A.ny - 1) s += (-c.n[lj]) * x(sy);
+    return s;
+}
+
+constexpr int N2 = H2 * W2, N1 = H1 * W1, R2 = (N2 + 255) / 256, R1 = (N1 + 255) / 256;
+
+// pre-smoothing (two sweeps from x = 0) and residual on one tile; all global loads
+// of a thread are issued before the first is used
 __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double *__restrict__ b,
                                                 double *__restrict__ x2g, double *__restrict__ rg, int tiles_x,
                                                 int ntiles) {
-    __shared__ double sb[H2 * W2], sw[H2 * W2], sx1[H2 * W2], sx2[H1 * W1];
+    __shared__ TileCoef cf;
+    __shared__ double sb[N2], swd[N2], sx2[N1];
     const int t = xcd_band(ntiles);
     if (t >= ntiles) return;  // uniform per block
     const int I0 = (t % tiles_x) * TX, J0 = (t / tiles_x) * TY;
-    for (int k = threadIdx.x; k < H2 * W2; k += blockDim.x) {
+    load_coef(A, I0, J0, cf);
+    double bv[R2], dv[R2];
+#pragma unroll
+    for (int m = 0; m < R2; m++) {
+        const int k = threadIdx.x + 256 * m;
         const int I = I0 - 2 + k % W2, J = J0 - 2 + k / W2;
-        double bb = 0, wd = 0, x1 = 0;
-        if (inside(A, I, J)) {
-            const int ip = J * A.nx + I;
-            bb = b[ip];
-            wd = w * (1.0 / p5_diag(A, I, J, ip));
-            x1 = wd * bb;
+        const bool in = k < N2 && inside(A, I, J);
+        const int ip = in ? J * A.nx + I : 0;
+        bv[m] = in ? b[ip] : 0.0;
+        dv[m] = in ? A.dinv[ip] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < R2; m++) {
+        const int k = threadIdx.x + 256 * m;
+        if (k < N2) {
+            sb[k] = bv[m];
+            swd[k] = w * dv[m];
         }
-        sb[k] = bb;
-        sw[k] = wd;
-        sx1[k] = x1;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < H1 * W1; k += blockDim.x) {
+    for (int k = threadIdx.x; k < N1; k += blockDim.x) {
         const int li = k % W1 + 1, lj = k / W1 + 1;
         const int I = I0 - 2 + li, J = J0 - 2 + lj;
         double v = 0;
         if (inside(A, I, J)) {
             const int c = lj * W2 + li;
-            const double s = p5_row(A, I, J, p5_diag(A, I, J, J * A.nx + I), [&](int o) { return sx1[c + o]; }, W2);
-            v = sx1[c] + sw[c] * (sb[c] - s);
+            const double d = t_diag(A, cf, li, lj, I, J, J * A.nx + I);
+            const double s = t_row(A, cf, li, lj, I, J, d, [&](int o) { return swd[c + o] * sb[c + o]; }, W2);
+            v = swd[c] * sb[c] + swd[c] * (sb[c] - s);
         }
         sx2[k] = v;
     }
@@ -164,56 +226,78 @@ __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double 
         if (!inside(A, I, J)) continue;
         const int ip = J * A.nx + I;
         const int c1 = (lj + 1) * W1 + li + 1, c2 = (lj + 2) * W2 + li + 2;
-        const double s = p5_row(A, I, J, p5_diag(A, I, J, ip), [&](int o) { return sx2[c1 + o]; }, W1);
+        const double d = t_diag(A, cf, li + 2, lj + 2, I, J, ip);
+        const double s = t_row(A, cf, li + 2, lj + 2, I, J, d, [&](int o) { return sx2[c1 + o]; }, W1);
         x2g[ip] = sx2[c1];
         rg[ip] = sb[c2] - s;
     }
 }
 
-// x = x2 + P xc, then two weighted-Jacobi sweeps, on one tile
-template <class V>
-__global__ void __launch_bounds__(256) k_p5_post(P5Dev A, double w, const int *__restrict__ prp,
-                                                 const int *__restrict__ pci, const V *__restrict__ pv,
-                                                 const double *__restrict__ xc, const double *__restrict__ x2,
-                                                 const double *__restrict__ b, double *__restrict__ xg, int tiles_x,
-                                                 int ntiles) {
-    __shared__ double sx[H2 * W2], sb[H1 * W1], sw[H1 * W1], sy[H1 * W1];
+// two weighted-Jacobi sweeps on one tile: x (halo 2) -> y (halo 1) -> xo (tile)
+__global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, double w, const double *__restrict__ x,
+                                                   const double *__restrict__ b, double *__restrict__ xo, int tiles_x,
+                                                   int ntiles) {
+    __shared__ TileCoef cf;
+    __shared__ double sx[N2], sb[N1], swd[N1];
     const int t = xcd_band(ntiles);
     if (t >= ntiles) return;
     const int I0 = (t % tiles_x) * TX, J0 = (t / tiles_x) * TY;
-    for (int k = threadIdx.x; k < H2 * W2; k += blockDim.x) {
+    load_coef(A, I0, J0, cf);
+    double xv[R2], bv[R1], dv[R1];
+#pragma unroll
+    for (int m = 0; m < R2; m++) {
+        const int k = threadIdx.x + 256 * m;
         const int I = I0 - 2 + k % W2, J = J0 - 2 + k / W2;
-        double v = 0;
-        if (inside(A, I, J)) {
-            const int ip = J * A.nx + I;
-            double s = 0;
-            for (int q = prp[ip]; q < prp[ip + 1]; q++) s += (double)pv[q] * xc[pci[q]];
-            v = x2[ip] + s;
-        }
-        sx[k] = v;
+        const bool in = k < N2 && inside(A, I, J);
+        xv[m] = in ? x[J * A.nx + I] : 0.0;
     }
-    for (int k = threadIdx.x; k < H1 * W1; k += blockDim.x) {
+#pragma unroll
+    for (int m = 0; m < R1; m++) {
+        const int k = threadIdx.x + 256 * m;
         const int I = I0 - 1 + k % W1, J = J0 - 1 + k / W1;
-        double bb = 0, wd = 0;
-        if (inside(A, I, J)) {
-            const int ip = J * A.nx + I;
-            bb = b[ip];
-            wd = w * (1.0 / p5_diag(A, I, J, ip));
+        const bool in = k < N1 && inside(A, I, J);
+        const int ip = in ? J * A.nx + I : 0;
+        bv[m] = in ? b[ip] : 0.0;
+        dv[m] = in ? A.dinv[ip] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < R2; m++) {
+        const int k = threadIdx.x + 256 * m;
+        if (k < N2) sx[k] = xv[m];
+    }
+#pragma unroll
+    for (int m = 0; m < R1; m++) {
+        const int k = threadIdx.x + 256 * m;
+        if (k < N1) {
+            sb[k] = bv[m];
+            swd[k] = w * dv[m];
         }
-        sb[k] = bb;
-        sw[k] = wd;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < H1 * W1; k += blockDim.x) {
-        const int li = k % W1 + 1, lj = k / W1 + 1;
-        const int I = I0 - 2 + li, J = J0 - 2 + lj;
+    // first sweep on the halo-1 region, kept in registers until every read of sx is done
+    double yv[R1];
+#pragma unroll
+    for (int m = 0; m < R1; m++) {
+        const int k = threadIdx.x + 256 * m;
         double v = 0;
-        if (inside(A, I, J)) {
-            const int c = lj * W2 + li;
-            const double s = p5_row(A, I, J, p5_diag(A, I, J, J * A.nx + I), [&](int o) { return sx[c + o]; }, W2);
-            v = sx[c] + sw[k] * (sb[k] - s);
+        if (k < N1) {
+            const int li = k % W1 + 1, lj = k / W1 + 1;
+            const int I = I0 - 2 + li, J = J0 - 2 + lj;
+            if (inside(A, I, J)) {
+                const int c = lj * W2 + li;
+                const double d = t_diag(A, cf, li, lj, I, J, J * A.nx + I);
+                const double s = t_row(A, cf, li, lj, I, J, d, [&](int o) { return sx[c + o]; }, W2);
+                v = sx[c] + swd[k] * (sb[k] - s);
+            }
         }
-        sy[k] = v;
+        yv[m] = v;
+    }
+    __syncthreads();
+    double *sy = sx;  // reuse: the halo-1 iterate in halo-1 layout
+#pragma unroll
+    for (int m = 0; m < R1; m++) {
+        const int k = threadIdx.x + 256 * m;
+        if (k < N1) sy[k] = yv[m];
     }
     __syncthreads();
     for (int k = threadIdx.x; k < TX * TY; k += blockDim.x) {
@@ -222,8 +306,9 @@ __global__ void __launch_bounds__(256) k_p5_post(P5Dev A, double w, const int *_
         if (!inside(A, I, J)) continue;
         const int ip = J * A.nx + I;
         const int c1 = (lj + 1) * W1 + li + 1;
-        const double s = p5_row(A, I, J, p5_diag(A, I, J, ip), [&](int o) { return sy[c1 + o]; }, W1);
-        xg[ip] = sy[c1] + sw[c1] * (sb[c1] - s);
+        const double d = t_diag(A, cf, li + 2, lj + 2, I, J, ip);
+        const double s = t_row(A, cf, li + 2, lj + 2, I, J, d, [&](int o) { return sy[c1 + o]; }, W1);
+        xo[ip] = sy[c1] + swd[c1] * (sb[c1] - s);
     }
 }
 
@@ -236,7 +321,7 @@ __global__ void __launch_bounds__(BS) k_p5_sweep(P5Dev A, double w, const double
     const int I = ip % A.nx, J = ip / A.nx;
     const double d = p5_diag(A, I, J, ip);
     const double s = p5_row(A, I, J, d, [&](int o) { return x[ip + o]; }, A.nx);
-    xn[ip] = x[ip] + w * (1.0 / d) * (b[ip] - s);
+    xn[ip] = x[ip] + w * A.dinv[ip] * (b[ip] - s);
 }
 
 __global__ void __launch_bounds__(BS) k_p5_apply(P5Dev A, const double *__restrict__ x, double *__restrict__ y,
@@ -260,6 +345,11 @@ PoissonStencil::PoissonStencil(const GridDev &g, double dt, bool centre_pin, hip
     const int m = std::max(nx_, ny_);
     hipLaunchKernelGGL(k_p5_coef, dim3((m + BS - 1) / BS), dim3(BS), 0, s, nx_, ny_, g.dx, g.dy, dt, cE_.p(), cW_.p(),
                        cN_.p(), cS_.p());
+    CUIBM_LAUNCH_CHECK();
+    dinv_.alloc((size_t)nx_ * ny_);
+    P5Dev d = dev();
+    d.dinv = nullptr;
+    hipLaunchKernelGGL(k_p5_dinv, dim3(std::min((nx_ * ny_ + BS - 1) / BS, 65536)), dim3(BS), 0, s, d, dinv_.p());
     CUIBM_LAUNCH_CHECK();
 }
 
@@ -293,19 +383,12 @@ void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, do
     CUIBM_LAUNCH_CHECK();
 }
 
-template <class V>
-void launch_p5_post(const P5Dev &A, double omega, const DCsr<V> &P, const double *xc, const double *x2,
-                    const double *b, double *x, hipStream_t s) {
+void launch_p5_sweep2(const P5Dev &A, double omega, const double *x, const double *b, double *xo, hipStream_t s) {
     int tx, nt;
     tiles(A, tx, nt);
-    hipLaunchKernelGGL(k_p5_post<V>, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, P.rp.p(), P.ci.p(), P.v.p(), xc,
-                       x2, b, x, tx, nt);
+    hipLaunchKernelGGL(k_p5_sweep2, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, x, b, xo, tx, nt);
     CUIBM_LAUNCH_CHECK();
 }
-template void launch_p5_post<double>(const P5Dev &, double, const DCsr<double> &, const double *, const double *,
-                                     const double *, double *, hipStream_t);
-template void launch_p5_post<float>(const P5Dev &, double, const DCsr<float> &, const double *, const double *,
-                                    const double *, double *, hipStream_t);
 
 void launch_p5_sweep(const P5Dev &A, double omega, const double *x, const double *b, double *xn, hipStream_t s) {
     const int nblk = (A.nx * A.ny + BS - 1) / BS;

@@ -106,7 +106,7 @@ class JacobiPrecond : public Preconditioner {
 // smoothed-aggregation AMG (setup on the host, V-cycle on the device)
 struct AmgParams {
     int precision = 64;   // 64 or 32
-    int coarse_max = 256;
+    int coarse_max = 2500;  // dense coarsest solve (one 25 MB FP32 / 50 MB FP64 matrix at most)
     int max_levels = 25;
     double omega = 0.8;
     int npre = 2, npost = 2;
@@ -122,6 +122,7 @@ struct P5Dev {
     int nx, ny;
     int centre;                        // cell whose diagonal is doubled besides cell 0, or -1
     const double *cE, *cW, *cN, *cS;   // coupling magnitudes; 0 where the neighbour is missing
+    const double *dinv;                // 1 / diagonal, per cell
 };
 
 class AmgPrecond : public Preconditioner {
@@ -139,6 +140,14 @@ class AmgPrecond : public Preconditioner {
     // run inside the V-cycle (for timing), and its algorithmic bytes
     void level0_sweep(const double *b, const double *x, double *xn, hipStream_t s);
     double level0_sweep_bytes() const;
+    // the two fused finest-level kernels of the stencil form (timing): pre-smoothing
+    // + residual (b -> x2, r) and prolongation + post-smoothing (x2, b -> x, with the
+    // current coarse correction); bytes are their algorithmic bytes
+    bool has_stencil_level() const { return fine_; }
+    void level0_pre(const double *b, double *x2, double *r, hipStream_t s);
+    void level0_post(double *x2, const double *b, double *x, hipStream_t s);
+    double level0_pre_bytes() const;
+    double level0_post_bytes() const;
 
   private:
     struct Impl;

@@ -432,6 +432,8 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
             launch_rhs1(u.p(), Nold.p(), t1.p(), t2.p(), nullptr, nullptr, grid(), bcs(), nu, dt, fadlunRhs(), s);
         else if (k == "vcycle") PC2->apply(rhs2.p(), t1.p(), s);
         else if (k == "jacobi0") amg2()->level0_sweep(rhs2.p(), pressure.p(), t1.p(), s);
+        else if (k == "vc_pre") amg2()->level0_pre(rhs2.p(), t1.p(), t2.p(), s);
+        else if (k == "vc_post") amg2()->level0_post(t2.p(), rhs2.p(), t1.p(), s);  // t2: scratch x2
         else throw Error("unknown kernel '" + k + "'", ERR_ARGUMENT);
     };
     if (k == "spmv_lhs2") b = csr_bytes(LHS2);
@@ -441,6 +443,9 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
     else if (k == "rhs1") b = 32.0 * numUV + 8.0 * (2 * nx + 2 * ny) + 4 * 8.0 * (2 * ny + 2 * nx);
     else if (k == "vcycle") b = PC2 ? PC2->bytes_per_apply() : 0;
     else if (k == "jacobi0") b = amg2()->level0_sweep_bytes();
+    else if (k == "vc_pre") b = amg2()->level0_pre_bytes();
+    else if (k == "vc_post") b = amg2()->level0_post_bytes();
+    if (k == "vc_post") launch_copy(t2.p(), pressure.p(), numP, s);
     once();  // warm-up
     hipEvent_t a, e;
     CUIBM_HIP(hipEventCreate(&a));

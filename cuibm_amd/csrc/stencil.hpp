@@ -11,10 +11,10 @@
 // y = A x is bit-identical to the CSR product.
 //
 // The V-cycle's finest level runs on this form with two fused LDS-tiled kernels
-// (pre-smoothing + residual, prolongation + post-smoothing) instead of seven CSR
-// launches; each block owns a 64 x 16 tile of cells and computes the intermediate
-// iterates on a halo, so the kernels read b (and x) once and write only their
-// results. Blocks are assigned tiles in XCD-contiguous bands (the hardware deals
+// (pre-smoothing + residual; both post-smoothing sweeps after a row-parallel
+// prolongation) instead of six CSR launches; each block owns a 64 x 16 tile of cells
+// and computes the intermediate iterates on a halo, so the kernels read b (and x)
+// once and write only their results. Blocks are assigned tiles in XCD-contiguous bands (the hardware deals
 // workgroups to the 8 XCDs round-robin), so the halo rows a block needs sit in its
 // own XCD's L2.
 #pragma once
@@ -32,22 +32,20 @@ class PoissonStencil : public LinOp {
     void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
               hipStream_t s) const override;
     double spmv_bytes() const override { return 16.0 * n() + 32.0 * (nx_ + ny_); }
-    P5Dev dev() const { return P5Dev{nx_, ny_, centre_, cE_.p(), cW_.p(), cN_.p(), cS_.p()}; }
+    P5Dev dev() const { return P5Dev{nx_, ny_, centre_, cE_.p(), cW_.p(), cN_.p(), cS_.p(), dinv_.p()}; }
     // y = A x without dot partials (tests)
     void apply(const double *x, double *y, hipStream_t s) const;
 
   private:
     int nx_, ny_, centre_;
-    DArray<double> cE_, cW_, cN_, cS_;
+    DArray<double> cE_, cW_, cN_, cS_, dinv_;
 };
 
 // finest-level V-cycle pieces (omega-weighted Jacobi, D = diag(A)):
-//   pre:  x1 = omega D^-1 b, x2 = x1 + omega D^-1 (b - A x1), r = b - A x2
-//   post: x = x2 + P xc, then two sweeps x <- x + omega D^-1 (b - A x)
+//   pre:    x1 = omega D^-1 b, x2 = x1 + omega D^-1 (b - A x1), r = b - A x2
+//   sweep2: two sweeps x <- x + omega D^-1 (b - A x) (after the prolongation)
 void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s);
-template <class V>
-void launch_p5_post(const P5Dev &A, double omega, const DCsr<V> &P, const double *xc, const double *x2,
-                    const double *b, double *x, hipStream_t s);
+void launch_p5_sweep2(const P5Dev &A, double omega, const double *x, const double *b, double *xo, hipStream_t s);
 // one sweep xn = x + omega D^-1 (b - A x) (timing of the finest-level smoother)
 void launch_p5_sweep(const P5Dev &A, double omega, const double *x, const double *b, double *xn, hipStream_t s);
 

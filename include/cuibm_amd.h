@@ -138,7 +138,9 @@ int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms);
  * (the same product fused with the two BiCGSTAB dot partials, as launched inside the
  * Poisson solve), "spmv_lhs1", "rhs1" (fused intermediate-velocity right-hand side),
  * "vcycle" (one Poisson preconditioner application), "jacobi0" (one weighted-Jacobi
- * sweep on the finest AMG level, as inside the V-cycle), "copy" (a copy of 8 Mi doubles,
+ * sweep on the finest AMG level), "vc_pre" / "vc_post" (the fused finest-level
+ * pre-smoothing + residual and prolongation + post-smoothing kernels of the stencil
+ * form, as launched inside the V-cycle), "copy" (a copy of 8 Mi doubles,
  * the calibration kernel for hardware byte counters) */
 int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms, double *bytes);
 

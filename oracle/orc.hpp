@@ -61,9 +61,13 @@ struct Amg {
     std::vector<double> coarse_inv;  // dense row-major inverse of the coarsest A
     double omega = 0.8;
     int npre = 2, npost = 2;
-    void setup(const Csr &A, int coarse_max = 256, int max_levels = 25);
+    // coarse_max <= 0: amg_default_coarse_max (2500; the tests lower it to reach
+    // several levels on small matrices)
+    void setup(const Csr &A, int coarse_max = 0, int max_levels = 25);
     void apply(const double *b, double *x);  // one V-cycle, zero initial guess
 };
+
+extern int amg_default_coarse_max;
 
 struct Precond {
     int type = P_NONE;

@@ -161,20 +161,24 @@ static bool dense_inverse(int n, std::vector<double> &M, std::vector<double> &In
             }
         const double d = 1.0 / M[(size_t)c * n + c];
         for (int k = 0; k < n; k++) { M[(size_t)c * n + k] *= d; Inv[(size_t)c * n + k] *= d; }
+        // Gauss-Jordan elimination of column c from every other row; rows are
+        // independent, and the columns of M left of c are already zero
+#pragma omp parallel for schedule(static)
         for (int r = 0; r < n; r++) {
             if (r == c) continue;
             const double f = M[(size_t)r * n + c];
             if (f == 0.0) continue;
-            for (int k = 0; k < n; k++) {
-                M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
-                Inv[(size_t)r * n + k] -= f * Inv[(size_t)c * n + k];
-            }
+            for (int k = c; k < n; k++) M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
+            for (int k = 0; k < n; k++) Inv[(size_t)r * n + k] -= f * Inv[(size_t)c * n + k];
         }
     }
     return true;
 }
 
+int amg_default_coarse_max = 2500;
+
 void Amg::setup(const Csr &A0, int coarse_max, int max_levels) {
+    if (coarse_max <= 0) coarse_max = amg_default_coarse_max;
     lv.clear();
     Csr A = A0;
     while (A.n > coarse_max && (int)lv.size() < max_levels - 1) {

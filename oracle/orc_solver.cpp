@@ -386,6 +386,9 @@ void orc_info(void *h, double *out) {
 }
 
 // standalone linear-algebra entry points used by the solver tests
+// coarsest size of the AMG hierarchies set up from now on (<= 0 restores 2500)
+void orc_set_amg_coarse_max(int n) { orc::amg_default_coarse_max = n > 0 ? n : 2500; }
+
 void *orc_csr(int n, const int *rp, const int *ci, const double *v) {
     auto *A = new orc::Csr();
     A->n = n;

@@ -75,6 +75,8 @@ def lib():
         L.orc_amg_levels.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]
         L.orc_amg_vcycle.restype = C.c_int
         L.orc_amg_vcycle.argtypes = [C.c_void_p, _DP, _DP]
+        L.orc_set_amg_coarse_max.argtypes = [C.c_int]
+        L.orc_set_amg_coarse_max.restype = None
         L.orc_amg_coarse_dense.restype = C.c_int
         L.orc_amg_coarse_dense.argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
         L.orc_amg_prolongator_dense.restype = C.c_int
@@ -165,6 +167,11 @@ def read_case(folder, overrides=None):
         else:
             sim[k] = v
     return case
+
+
+def set_amg_coarse_max(n):
+    """coarsest AMG level size for hierarchies set up from now on (n <= 0: 2500)"""
+    lib().orc_set_amg_coarse_max(int(n))
 
 
 class Oracle:

@@ -217,7 +217,8 @@ def test_bicgstab_matches_dense_solve(precond):
     xe = np.linalg.solve(A.toarray(), b)
     assert np.linalg.norm(x - xe) <= 1e-11 * np.linalg.norm(xe)
     assert np.linalg.norm(b - A @ x) <= 1e-13 * np.linalg.norm(b) * 1.0001
-    assert 0 < it < 100
+    # a hierarchy of one level is an exact (dense) inverse: converged at the first half step
+    assert it < 100 and (it > 0 or precond == "SMOOTHED_AGGREGATION")
 
 
 def test_bicgstab_monitor_semantics():
@@ -236,7 +237,15 @@ def _poisson_lhs2(n=32):
     return _csr(_cavity(n), "LHS2")
 
 
-def test_amg_galerkin_identity():
+@pytest.fixture
+def small_coarse():
+    """a coarsest level of <= 256 unknowns, so that a 32x32 problem has several levels"""
+    po.set_amg_coarse_max(256)
+    yield
+    po.set_amg_coarse_max(0)
+
+
+def test_amg_galerkin_identity(small_coarse):
     A = _poisson_lhs2(32)
     M = po.OrcCsr(A)
     sizes, nnz = M.amg_levels()
@@ -248,7 +257,7 @@ def test_amg_galerkin_identity():
     assert nnz[1] == np.count_nonzero(Ac)
 
 
-def test_amg_vcycle_is_symmetric_and_contracts():
+def test_amg_vcycle_is_symmetric_and_contracts(small_coarse):
     """equal pre/post Jacobi sweeps and R = P^T make the V-cycle a symmetric operator;
     as a stationary iteration it reduces the error on the pinned Poisson problem"""
     A = _poisson_lhs2(32)
