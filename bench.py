@@ -22,8 +22,10 @@ about 55 steps in this method (GPU and CPU restatement alike); the default windo
 
 Prints one JSON line. ``roofline`` reports the dominant kernel of the step (the
 fused finest-level pre-smoothing + residual kernel of the Poisson V-cycle, one per
-V-cycle and two V-cycles per BiCGSTAB iteration) timed with HIP events over
-back-to-back launches on the live hierarchy; ``traffic`` is the
+V-cycle and two V-cycles per BiCGSTAB iteration), timed with HIP events around each
+of its launches inside a few further live time steps after the timed window (the
+timed window itself carries no probes); ``kernels`` lists back-to-back timings of
+single kernels on the live inputs for comparison; ``traffic`` is the
 HBM byte count per launch of that kernel from hardware counters, read from the
 committed summary under profiles/ when present. ``cpu_baseline`` is the FP64 C++
 CPU restatement of the same step (the oracle) timed on this host over a short
@@ -52,7 +54,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(cu, case, amg_precision, warmup, steps, kernel, reps):
+def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5):
     """initialise, warm up, time `steps` steps; return a dict of figures"""
     t0 = time.perf_counter()
     s = cu.Solver(case, "-amgPrecision", str(amg_precision), diagnostics=False)
@@ -87,20 +89,24 @@ def run_form(cu, case, amg_precision, warmup, steps, kernel, reps):
         out["velocity_bytes_per_step"]
     out["hbm_gbps_step"] = out["step_bytes"] / (out["ms_per_step"] * 1e-3) / 1e9
     if kernel:
+        # roofline kernel: HIP events around each of its launches inside `probe_steps`
+        # further live time steps (after the timed window, which stays unprobed)
+        kms, kbytes = s.time_kernel("live:" + kernel, probe_steps)
+        out["kernel"] = {"name": kernel, "ms": kms, "bytes": kbytes, "steps": probe_steps}
         out["kernels"] = {}
-        for k in dict.fromkeys([kernel] + EXTRA_KERNELS):
-            kms, kbytes = s.time_kernel(k, reps)
-            out["kernels"][k] = {"us": kms * 1e3, "bytes": kbytes, "gbps": kbytes / (kms * 1e-3) / 1e9}
-        out["kernel"] = {"name": kernel, "ms": out["kernels"][kernel]["us"] * 1e-3, "bytes": out["kernels"][kernel]["bytes"]}
+        if reps > 0:  # back-to-back launches of single kernels on the live inputs
+            for k in dict.fromkeys([kernel] + EXTRA_KERNELS):
+                kms, kbytes = s.time_kernel(k, reps)
+                out["kernels"][k] = {"us": kms * 1e3, "bytes": kbytes, "gbps": kbytes / (kms * 1e-3) / 1e9}
     s.close()
     return out
 
 
-def read_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed counter summary"""
+def read_profile(key):
+    """a figure of the roofline kernel from the committed counter / trace summary"""
     try:
         with open(TRAFFIC_FILE) as f:
-            m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", f.read())
+            m = re.search(key + r"\s*[:=]\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
         return None
@@ -131,12 +137,15 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--kernel", default="vc_pre", help="roofline kernel (see cuibm_time_kernel)")
-    ap.add_argument("--kernel-reps", type=int, default=100)
+    ap.add_argument("--kernel-reps", type=int, default=100,
+                    help="back-to-back launches per kernel for the kernel table (0: skip the table)")
+    ap.add_argument("--probe-steps", type=int, default=5,
+                    help="live time steps after the timed window over which the roofline kernel is timed")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-hierarchy comparison run")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="initialise the case and launch only the roofline kernel and the copy calibration kernel "
-                         "(for hardware-counter runs); prints their HIP-event timings")
+                    help="initialise the case, run two time steps and the copy calibration kernel "
+                         "(for hardware-counter runs)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -149,17 +158,20 @@ def main():
 
     case = os.path.join(cu.CASES, CASE)
     if args.kernels_only:
+        # two live time steps (their kernels are the ones the counters should see), then
+        # the copy calibration kernel
         s = cu.Solver(case, diagnostics=False)
         s.initialise()
+        s.step_time(2)
         out = {}
-        for k in dict.fromkeys([args.kernel, "spmv_dot_lhs2", "copy"]):
+        for k in ("copy",):
             ms, b = s.time_kernel(k, args.kernel_reps)
             out[k] = {"us": ms * 1e3, "bytes": b, "gbps": b / (ms * 1e-3) / 1e9}
         s.close()
         print(json.dumps(out), flush=True)
         return
     log(f"[bench] mixed form (FP32 AMG matrix values): warmup {args.warmup}, steps {args.steps}")
-    mixed = run_form(cu, case, 32, args.warmup, args.steps, args.kernel, args.kernel_reps)
+    mixed = run_form(cu, case, 32, args.warmup, args.steps, args.kernel, args.kernel_reps, args.probe_steps)
     log(f"[bench] mixed: {mixed['steps_per_s']:.2f} steps/s, Poisson its/step {mixed['poisson_iters_per_step']:.1f}")
     fp64 = None
     if not args.no_fp64:
@@ -207,9 +219,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic(),
+            "traffic": read_profile("traffic_bytes_per_launch"),
+            "rocprof_us_per_launch": read_profile("rocprof_us_per_launch"),
             "bytes_per_launch": k["bytes"],
             "us_per_launch": round(k["ms"] * 1e3, 2),
+            "timing": f"HIP events around each in-step launch over {k['steps']} live time steps after the timed window",
         },
         "kernels": {n: {kk: round(vv, 2) for kk, vv in d.items()} for n, d in mixed["kernels"].items()},
         "cpu_baseline": cpu,

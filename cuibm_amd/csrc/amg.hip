@@ -137,16 +137,30 @@ void launch_csr_op(int lpr, const DCsr<V> &A, const double *x, const double *b, 
     CUIBM_LAUNCH_CHECK();
 }
 
-// x = Ainv b on the coarsest level: one wavefront per row, lanes across the row
-// (row-major Ainv, coalesced), fixed shuffle-tree sum
+// x = Ainv b on the coarsest level: b staged in LDS (n doubles of dynamic LDS), one
+// wavefront per row with lanes across the row-major inverse (coalesced), four
+// independent partial sums per lane combined in a fixed order, then a fixed shuffle
+// tree across the wavefront
 template <class V>
 __global__ void __launch_bounds__(256) k_dense(int n, const V *__restrict__ Ainv, const double *__restrict__ b,
                                                double *__restrict__ x) {
+    extern __shared__ double sb[];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = b[j];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;  // uniform per wavefront
-    double s = 0;
-    for (int j = lane; j < n; j += 64) s += (double)Ainv[(size_t)row * n + j] * b[j];
+    if (row >= n) return;  // uniform per wavefront, after the only barrier
+    const V *a = Ainv + (size_t)row * n;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int j = lane;
+    for (; j + 192 < n; j += 256) {
+        s0 += (double)a[j] * sb[j];
+        s1 += (double)a[j + 64] * sb[j + 64];
+        s2 += (double)a[j + 128] * sb[j + 128];
+        s3 += (double)a[j + 192] * sb[j + 192];
+    }
+    for (; j < n; j += 64) s0 += (double)a[j] * sb[j];
+    double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if (lane == 0) x[row] = s;
@@ -171,6 +185,7 @@ struct AmgDev {
     AmgParams prm;
     bool use_fine = false;  // finest level in stencil form (fused kernels)
     P5Dev fine{};
+    KernelProbe *probe = nullptr;  // events around the finest-level pre-smoothing launches
 
     void build(const std::vector<HostAmgLevel> &h, hipStream_t s) {
         lv.resize(h.size());
@@ -204,7 +219,7 @@ struct AmgDev {
             L.r.alloc(L.n);
         }
         nc = h.back().A.n;
-        if (nc > 8192) throw Error("AMG coarsening stalled at " + std::to_string(nc) + " unknowns", ERR_GENERIC);
+        if (nc > 6144) throw Error("AMG coarsening stalled at " + std::to_string(nc) + " unknowns", ERR_GENERIC);
         const std::vector<double> inv = dense_inverse(h.back().A);
         std::vector<V> invt(inv.begin(), inv.end());
         coarse_inv.upload(invt, s);
@@ -217,14 +232,16 @@ struct AmgDev {
         const int n = L.n;
         const double w = prm.omega;
         if (l + 1 == lv.size()) {
-            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), 0, s, n, coarse_inv.p(), b, x);
+            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), n * sizeof(double), s, n, coarse_inv.p(), b, x);
             CUIBM_LAUNCH_CHECK();
             return;
         }
         if (l == 0 && use_fine && prm.npre == 2 && prm.npost == 2) {
             // stencil finest level: x2 -> xt and r -> r, restriction, coarse cycle,
             // prolongation into xt, both post-sweeps straight into x
+            if (probe) probe->mark(s);
             launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s);
+            if (probe) probe->mark(s);
             AmgLevelDev<V> &C = lv[1];
             launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
             vcycle(1, C.b.p(), C.x.p(), s);
@@ -272,7 +289,10 @@ struct AmgPrecond::Impl {
     AmgDev<double> d64;
     AmgDev<float> d32;
     std::vector<HostAmgLevel> host;
+    KernelProbe probe;
 };
+
+KernelProbe &AmgPrecond::pre_probe() { return im_->probe; }
 
 AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine)
     : im_(new Impl), prm_(p) {
@@ -282,6 +302,8 @@ AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P
         if (fine->nx * fine->ny != A.n) throw Error("stencil and matrix sizes differ", ERR_ARGUMENT);
         fine_dev_ = *fine;
     }
+    im_->d32.probe = &im_->probe;
+    im_->d64.probe = &im_->probe;
     if (p.precision == 32) {
         im_->d32.prm = p;
         im_->d32.use_fine = fine_;

@@ -3,6 +3,7 @@
 #pragma once
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "dev.hpp"
 
@@ -112,6 +113,45 @@ struct AmgParams {
     int npre = 2, npost = 2;
 };
 
+// HIP events around the launches of one kernel inside the live solve: mark() before
+// and after each launch while enabled; total_ms() waits and sums the pairs
+class KernelProbe {
+  public:
+    ~KernelProbe() {
+        for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+    }
+    void enable(bool on) {
+        on_ = on;
+        used_ = 0;
+    }
+    bool enabled() const { return on_; }
+    void mark(hipStream_t s) {
+        if (!on_) return;
+        if (used_ == ev_.size()) {
+            hipEvent_t e;  // timestamps only: no system-scope release fence at the record
+            CUIBM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            ev_.push_back(e);
+        }
+        CUIBM_HIP(hipEventRecord(ev_[used_++], s));
+    }
+    int launches() const { return (int)(used_ / 2); }
+    double total_ms() const {
+        double t = 0;
+        for (size_t i = 0; i + 1 < used_; i += 2) {
+            CUIBM_HIP(hipEventSynchronize(ev_[i + 1]));
+            float ms = 0;
+            CUIBM_HIP(hipEventElapsedTime(&ms, ev_[i], ev_[i + 1]));
+            t += ms;
+        }
+        return t;
+    }
+
+  private:
+    std::vector<hipEvent_t> ev_;
+    size_t used_ = 0;
+    bool on_ = false;
+};
+
 struct AmgHierarchyInfo {
     int levels = 0;
     std::vector<int> n, nnz, nnzP;
@@ -144,6 +184,8 @@ class AmgPrecond : public Preconditioner {
     // + residual (b -> x2, r) and prolongation + post-smoothing (x2, b -> x, with the
     // current coarse correction); bytes are their algorithmic bytes
     bool has_stencil_level() const { return fine_; }
+    // events around every finest-level pre-smoothing launch of the live V-cycles
+    KernelProbe &pre_probe();
     void level0_pre(const double *b, double *x2, double *r, hipStream_t s);
     void level0_post(double *x2, const double *b, double *x, hipStream_t s);
     double level0_pre_bytes() const;

@@ -412,6 +412,19 @@ double NavierStokesSolver::stepBytes() const {
 
 int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, double *bytes) {
     hipStream_t s = stream_;
+    if (k == "live:vc_pre") {  // the finest-level pre-smoothing launches of `reps` live time steps
+        AmgPrecond *a = amg2();
+        if (!a->has_stencil_level()) throw Error("the finest AMG level is not in stencil form", ERR_ARGUMENT);
+        KernelProbe &pr = a->pre_probe();
+        pr.enable(true);
+        for (int r = 0; r < reps; r++) stepTime();
+        const double t = pr.total_ms();
+        const int nl = pr.launches();
+        pr.enable(false);
+        if (ms) *ms = nl > 0 ? t / nl : 0.0;
+        if (bytes) *bytes = a->level0_pre_bytes();
+        return nl;
+    }
     DArray<double> t1(std::max(numUV, numP)), t2(std::max(numUV, numP)), part(2 * RED_GRID);
     // calibration copy: 64 MiB per array, larger than the L2 of every XCD together, so
     // that back-to-back launches cannot serve reads from L2

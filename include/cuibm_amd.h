@@ -141,7 +141,10 @@ int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms);
  * sweep on the finest AMG level), "vc_pre" / "vc_post" (the fused finest-level
  * pre-smoothing + residual and prolongation + post-smoothing kernels of the stencil
  * form, as launched inside the V-cycle), "copy" (a copy of 8 Mi doubles,
- * the calibration kernel for hardware byte counters) */
+ * the calibration kernel for hardware byte counters). "live:vc_pre" instead runs
+ * `reps` time steps and returns the mean duration of the finest-level pre-smoothing
+ * launches inside them (HIP events around each launch); the return value is then the
+ * number of launches timed. */
 int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms, double *bytes);
 
 /* block until all work on the current device has finished (hipDeviceSynchronize);
