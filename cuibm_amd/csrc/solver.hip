@@ -519,16 +519,24 @@ void luo_base::initialise() {
     for (int i = 0; i < n; i++) { mx += bX[i]; my += bY[i]; }
     mx /= n;
     my /= n;
+    // luo_base.cu:44-85: prescribed in-line oscillation x = xC sin(2 pi f t + xPhase),
+    // u = uC cos(2 pi f t + uPhase) about the centroid; the points keep their offsets
     const BodyDesc &b0 = db->bodies[0];
+    moving = b0.xCoefficient != 0 || b0.uCoefficient != 0;
+    if (moving && db->solverType == FADLUN)
+        throw Error("a moving body with the modified Fadlun method is not supported (its LHS is not rebuilt)",
+                    ERR_UNSUPPORTED);
     const double t = dt * timeStep;
     const double xnew = b0.xCoefficient * std::sin(2 * M_PI * b0.xfrequency * t + b0.xPhase);
     const double unew = b0.uCoefficient * std::cos(2 * M_PI * b0.xfrequency * t + b0.uPhase);
     const double uoldd = b0.uCoefficient * std::cos(2 * M_PI * b0.xfrequency * (t - dt) + b0.uPhase);
-    if (b0.xCoefficient != 0 || b0.uCoefficient != 0)
-        throw Error("prescribed body oscillation is not supported yet", ERR_UNSUPPORTED);
     midX = xnew;
     midY = my;
-    for (int i = 0; i < n; i++) bX[i] = midX + (bX[i] - mx);
+    bDX.resize(n);
+    for (int i = 0; i < n; i++) {
+        bDX[i] = bX[i] - mx;
+        bX[i] = midX + bDX[i];
+    }
     bUB.assign(n, unew);
     bVB.assign(n, 0.0);
     uBk = uoldd;
@@ -687,11 +695,30 @@ void luo_base::_update_body() {
 void luo_base::_post_step() { timeStep++; }
 
 // luo_base.cu:154-174: force, then the prescribed motion (a static body here)
-void luo_base::moveBody() { calculateForce(); }
+void luo_base::moveBody() {
+    calculateForce();
+    if (!moving) return;
+    // luo_base/move.cu:11-36 (set_movement) at t = dt * timeStep, before the step counter advances
+    const BodyDesc &b0 = db->bodies[0];
+    const double t = dt * timeStep;
+    const double xnew = b0.xCoefficient * std::sin(2 * M_PI * b0.xfrequency * t + b0.xPhase);
+    const double unew = b0.uCoefficient * std::cos(2 * M_PI * b0.xfrequency * t + b0.uPhase);
+    midX = xnew;
+    for (size_t i = 0; i < bX.size(); i++) {
+        bX[i] = midX + bDX[i];
+        bUB[i] = unew;
+    }
+    dbx.upload(bX, stream_);
+    dbu.upload(bUB, stream_);
+}
 
-// luo_base.cu:137-147: for a static body the re-tag reproduces the same tags, so it
-// is only run when the body has moved
-void luo_base::updateSolver() {}
+// luo_base.cu:137-147: bounding box and tags for the moved body. For a static body the
+// reference's per-step re-tag reproduces the same tags, so it is only run after a move.
+void luo_base::updateSolver() {
+    if (!moving) return;
+    boundingBox();
+    tagPoints();
+}
 
 void luo_base::calculateForce() {
     launch_force(u.p(), uold.p(), pressure.p(), ghostTagsUV.p(), grid(), nu, dt, startI, startJ, numCellsX, numCellsY,

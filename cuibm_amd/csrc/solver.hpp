@@ -135,6 +135,8 @@ class luo_base : public NavierStokesSolver {
   protected:
     // body state (src/bodies.h)
     std::vector<double> bX, bY, bUB, bVB;  // host copies of the body points / velocities
+    std::vector<double> bDX;               // point offsets from the body centre (x)
+    bool moving = false;                   // prescribed oscillation (nonzero x or u coefficient)
     double midX = 0, midY = 0, uB0 = 0, vB0 = 0, uBk = 0, vBk = 0;
     int startI = 0, startJ = 0, numCellsX = 0, numCellsY = 0;
     double forceX = 0, forceY = 0, fxx = 0, fxy = 0, fxu = 0;

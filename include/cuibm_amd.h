@@ -54,7 +54,9 @@ int cuibm_create(const char *case_folder, int argc, const char *const *argv, cui
  * fadlunModified.cu:28-43): fields, grid, bodies, tags, matrices, preconditioners */
 int cuibm_initialise(cuibm_solver *s);
 
-/* NavierStokesSolver::stepTime() (NavierStokesSolver.cu:163-188, luo_base.cu:208-233) */
+/* NavierStokesSolver::stepTime() (NavierStokesSolver.cu:163-188, luo_base.cu:208-233);
+ * a LUO body with a prescribed oscillation is moved and re-tagged at the end of the
+ * step (luo_base.cu:137-174, luo_base/move.cu:11-36) */
 int cuibm_step_time(cuibm_solver *s);
 
 /* NavierStokesSolver::writeData() (NavierStokesSolver.cu:356-363, luoIBM.cu:69-80,

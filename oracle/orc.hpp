@@ -96,11 +96,15 @@ struct Params {
     double tol1 = 1e-5, tol2 = 1e-5;
     int maxit1 = 10000, maxit2 = 20000;
     double scaleCV = 2.0;
+    // prescribed in-line oscillation of the body (src/io/parseBodiesFile.cu:60-104)
+    double xCoeff = 0, uCoeff = 0, xPhase = 0, uPhase = 0, xfreq = 0;
 };
 
 struct Body {
     int n = 0;
     std::vector<double> x, y, uB, vB, uB0, vB0, uBk, vBk;
+    std::vector<double> dx;  // point offsets from the body centre (x)
+    bool moving = false;
     double midX = 0, midY = 0;
     int startI = 0, startJ = 0, numCellsX = 0, numCellsY = 0;
     double forceX = 0, forceY = 0;
@@ -158,6 +162,7 @@ struct Solver {
     void step();
     void step_ns();
     void step_luo();
+    void move_body();
     void step_fadlun();
     std::vector<double> *field(const std::string &name);
     std::vector<int> *ifield(const std::string &name);

@@ -81,10 +81,22 @@ void Solver::init(const Params &Pin, const Grid &Gin, const std::vector<double> 
         for (int i = 0; i < B.n; i++) { mx += B.x[i]; my += B.y[i]; }
         mx /= B.n;
         my /= B.n;
-        // luo_base.cu:70-83 with zero oscillation coefficients: midX = 0, x_i = 0 + (x_i - mean)
-        B.midX = 0.0;
+        // luo_base.cu:44-85: position and velocity of the prescribed oscillation at t = dt * timeStep,
+        // the previous velocity at t - dt; the points keep their offsets from the centroid
+        B.moving = P.xCoeff != 0 || P.uCoeff != 0;
+        const double t = P.dt * timeStep;
+        const double xnew = P.xCoeff * std::sin(2 * M_PI * P.xfreq * t + P.xPhase);
+        const double unew = P.uCoeff * std::cos(2 * M_PI * P.xfreq * t + P.uPhase);
+        const double uoldd = P.uCoeff * std::cos(2 * M_PI * P.xfreq * (t - P.dt) + P.uPhase);
+        B.midX = xnew;
         B.midY = my;
-        for (int i = 0; i < B.n; i++) B.x[i] = B.midX + (B.x[i] - mx);
+        B.dx.resize(B.n);
+        for (int i = 0; i < B.n; i++) {
+            B.dx[i] = B.x[i] - mx;
+            B.x[i] = B.midX + B.dx[i];
+        }
+        B.uB.assign(B.n, unew);
+        B.uBk.assign(B.n, uoldd);
         tagPoints();
         if (P.solver_type == LUO) set_inside(u);  // luoIBM.cu:42
     }
@@ -150,9 +162,27 @@ void Solver::step_luo() {
     project_nobody();
     gn_velocity(u, true);
     set_inside(u);
-    // luo_base.cu:154-174 (static body: the re-tag of updateSolver reproduces the tags)
+    // luo_base.cu:154-174 and 137-147: force, prescribed motion, bounding box and re-tag
+    // (for a static body the re-tag reproduces the tags and is skipped)
     force();
+    if (B.moving) {
+        move_body();
+        boundingBox();
+        tagPoints();
+    }
     timeStep++;
+}
+
+// luo_base/move.cu:11-36 (set_movement) at t = dt * timeStep, before the step counter advances
+void Solver::move_body() {
+    const double t = P.dt * timeStep;
+    const double xnew = P.xCoeff * std::sin(2 * M_PI * P.xfreq * t + P.xPhase);
+    const double unew = P.uCoeff * std::cos(2 * M_PI * P.xfreq * t + P.uPhase);
+    B.midX = xnew;
+    for (int i = 0; i < B.n; i++) {
+        B.uB[i] = unew;
+        B.x[i] = B.midX + B.dx[i];
+    }
 }
 
 // luo_base.cu:208-285 with fadlunModified.cu:97-124
@@ -257,6 +287,7 @@ struct orc_params {
     double scaleCV;
     int nbody;
     const double *bx, *by;
+    double xCoeff, uCoeff, xPhase, uPhase, xfreq;
 };
 
 void *orc_create(const orc_params *q) {
@@ -277,6 +308,11 @@ void *orc_create(const orc_params *q) {
     P.maxit1 = q->maxit1;
     P.maxit2 = q->maxit2;
     P.scaleCV = q->scaleCV;
+    P.xCoeff = q->xCoeff;
+    P.uCoeff = q->uCoeff;
+    P.xPhase = q->xPhase;
+    P.uPhase = q->uPhase;
+    P.xfreq = q->xfreq;
     orc::Grid G;
     G.nx = q->nx;
     G.ny = q->ny;

@@ -37,6 +37,8 @@ class OrcParams(C.Structure):
         ("tol1", C.c_double), ("tol2", C.c_double),
         ("maxit1", C.c_int), ("maxit2", C.c_int),
         ("scaleCV", C.c_double), ("nbody", C.c_int), ("bx", _DP), ("by", _DP),
+        ("xCoeff", C.c_double), ("uCoeff", C.c_double), ("xPhase", C.c_double), ("uPhase", C.c_double),
+        ("xfreq", C.c_double),
     ]
 
 
@@ -148,15 +150,21 @@ def read_case(folder, overrides=None):
         grid[node["direction"]] = (x, dx)
         grid["mid_h"] = float(node["mid_h"])
     bodies = []
+    motion = {"xCoeff": 0.0, "uCoeff": 0.0, "xPhase": 0.0, "uPhase": 0.0, "xfreq": 0.0}
     bpath = os.path.join(folder, "bodies.yaml")
     if os.path.exists(bpath):
-        for node in _load(folder, "bodies.yaml") or []:
+        for ib, node in enumerate(_load(folder, "bodies.yaml") or []):
+            if ib == 0:  # the motion of the first body drives all points (src/bodies.cu:27-35)
+                for key, k in (("xCoefficient", "xCoeff"), ("uCoefficient", "uCoeff"), ("xPhase", "xPhase"),
+                               ("uPhase", "uPhase"), ("xfrequency", "xfreq")):
+                    if key in node:
+                        motion[k] = float(node[key])
             if node.get("type") == "circle":
                 o = node["circleOptions"]
                 self_size = bool(o[4]) if len(o) > 4 else False
                 mid_h = float(o[5]) if len(o) > 5 else 0.0
                 bodies.append(circle(float(o[0]), float(o[1]), float(o[2]), int(o[3]), self_size, mid_h))
-    case = {"sim": sim, "flow": flow, "grid": grid, "bodies": bodies}
+    case = {"sim": sim, "flow": flow, "grid": grid, "bodies": bodies, "motion": motion}
     for k, v in (overrides or {}).items():
         if k in ("nu",):
             flow[k] = v
@@ -203,6 +211,9 @@ class Oracle:
         p.maxit1 = s["velocity"]["maxIterations"]
         p.maxit2 = s["Poisson"]["maxIterations"]
         p.scaleCV = float(s["scaleCV"])
+        m = case.get("motion", {})
+        p.xCoeff, p.uCoeff = m.get("xCoeff", 0.0), m.get("uCoeff", 0.0)
+        p.xPhase, p.uPhase, p.xfreq = m.get("xPhase", 0.0), m.get("uPhase", 0.0), m.get("xfreq", 0.0)
         if case["bodies"]:
             bx = np.concatenate([b[0] for b in case["bodies"]])
             by = np.concatenate([b[1] for b in case["bodies"]])

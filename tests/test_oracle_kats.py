@@ -388,3 +388,28 @@ def test_cv_force_vanishes_for_uniform_flow():
     I, J, W, H = (int(f[k]) for k in ("startI", "startJ", "numCellsX", "numCellsY"))
     assert np.isclose(f["fxx"], 2.0 * (x[I + W] - x[I]) * dy[J:J + H].sum(), rtol=1e-13)
     assert f["fxu"] == 0 and abs(f["forceY"]) < 1e-12
+
+
+def test_oscillating_body_motion_matches_reference_run():
+    """The reference's committed run of the in-line oscillating cylinder
+    (validation/osc/static/external, fixture written by tests/golden/make_osc_fixtures.py)
+    fixes the body motion after every step: x = -sin(2 pi t)/(2 pi) and u = -cos(2 pi t)
+    evaluated at t = dt * (step - 1), because the move at the end of a step uses the step
+    counter before it advances (luo_base.cu:255-264, luo_base/move.cu:11-36). Printed to
+    six significant digits. The velocity solve's iteration counts are in the same file and
+    must agree over the first 40 steps (2 per step)."""
+    with open(os.path.join(GOLDEN, "osc_reference_series.json")) as f:
+        ref = json.load(f)
+    o = po.Oracle(po.read_case(case("oscCylinderRe100_luo")))
+    n = 40
+    mid, its = [], []
+    for _ in range(n):
+        o.step(1)
+        i = o.info()
+        mid.append(i["midX"])
+        its.append(int(i["iter1"]))
+    rx = np.array(ref["midX"][:n])
+    assert np.all(np.abs(np.array(mid) - rx) <= 6e-6 * np.abs(rx) + 1e-15)
+    # an off-by-one in the motion's time would shift x by dt * u = 1e-3 per step
+    assert abs(mid[5] - ref["midX"][4]) > 1e-4
+    assert its == ref["iter_velocity"][:n]
