@@ -34,7 +34,12 @@ INFO_KEYS = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "
              "startI", "startJ", "numCellsX", "numCellsY", "midX", "midY", "amg_levels", "nx", "ny",
              "tol1", "tol2", "n_ghost_uv", "n_hybrid_uv", "n_ghost_p", "n_hybrid_p", "n_solid",
              "iter1_total", "iter2_total", "velocity_bytes_total", "poisson_bytes_total", "poisson_solves",
-             "krylov_restarts"]
+             "krylov_restarts",
+             # Luo embedded assembly, slot anomalies summed over all assemblies (LHS1, then LHS2):
+             # 5-point slots a ghost row leaves unwritten, hybrid tail slots never written,
+             # tail slots written twice, interpolation corners outside the 3x3 stencil
+             "emb1_unwritten", "emb1_tail_unwritten", "emb1_tail_multi", "emb1_dropped",
+             "emb2_unwritten", "emb2_tail_unwritten", "emb2_tail_multi", "emb2_dropped"]
 
 
 class CuibmError(RuntimeError):
@@ -193,8 +198,8 @@ class Solver:
         _check(lib().cuibm_set_field(self._h, name.encode(), a.ctypes.data_as(C.c_void_p), a.size))
 
     def info(self):
-        out = np.zeros(32)
-        _check(lib().cuibm_info(self._h, _dp(out), 32))
+        out = np.zeros(40)
+        _check(lib().cuibm_info(self._h, _dp(out), 40))
         return dict(zip(INFO_KEYS, out[: len(INFO_KEYS)]))
 
     def bytes(self, which=0):

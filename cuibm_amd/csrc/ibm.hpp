@@ -59,6 +59,7 @@ void launch_zero_rows(int *tags, const BoxDev &box, int rowlen, int kmax, int of
 struct Stencils {
     int n = 0;
     DArray<int> node, idx;        // node[n], idx[4n]
+    DArray<int> idx0;             // idx before the boundary-corner substitution (4n)
     DArray<double> coef;          // coef[4n]
     DArray<double> nrm;           // Neumann corner normals (nx, ny) per corner: 8n
     DArray<double> extra;         // per node: alpha (hybrid nodes) or dist/close-normal (pressure ghost: 3n)

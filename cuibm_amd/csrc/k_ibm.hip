@@ -31,16 +31,21 @@ int grid_for(long n) {
     return (int)(b < 65536 ? b : 65536);
 }
 
+// squares are products: pow(x, 2) in the reference
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
 // foot of the perpendicular from (px,py) to the body line through points k,l
-// (tagPoints.cu:122-133)
+// (tagPoints.cu:122-133). The reference forms the distance along the line as
+// a / tan(theta) with theta = acos(c), a = b sin(theta), c = (p^2 + b^2 - o^2) / (2 p b);
+// that is b cos(theta) = (p^2 + b^2 - o^2) / (2 p), evaluated here in that form so that
+// only correctly rounded operations are involved and a CPU evaluation agrees bit for bit.
 __device__ void foot(const double *bx, const double *by, int k, int l, double px, double py, double &fx, double &fy) {
-    const double p = sqrt(pow((bx[l] - bx[k]), 2) + pow((by[l] - by[k]), 2));
-    const double o = sqrt(pow((bx[l] - px), 2) + pow((by[l] - py), 2));
-    const double b = sqrt(pow((px - bx[k]), 2) + pow((py - by[k]), 2));
-    const double th = acos((p * p + b * b - o * o) / (2 * p * b));
-    const double a = sin(th) * b;
-    fy = by[k] + (by[l] - by[k]) / p * a / tan(th);
-    fx = bx[k] + (bx[l] - bx[k]) / p * a / tan(th);
+    const double p = sqrt(sq((bx[l] - bx[k])) + sq((by[l] - by[k])));
+    const double o = sqrt(sq((bx[l] - px)) + sq((by[l] - py)));
+    const double b = sqrt(sq((px - bx[k])) + sq((py - by[k])));
+    const double d = (p * p + b * b - o * o) / (2 * p);
+    fy = by[k] + (by[l] - by[k]) / p * d;
+    fx = bx[k] + (bx[l] - bx[k]) / p * d;
 }
 
 __device__ __forceinline__ int seg_k(int l, int n) { return l == 0 ? n - 1 : l - 1; }
@@ -457,7 +462,7 @@ __device__ __forceinline__ CompGeom comp_of(const GridDev &g, int node) {
 }
 
 // interpolateVelocityToGhostNodeX/Y weights (biLinearInterpolation.cu:14-353)
-__global__ void k_gn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, double *coef) {
+__global__ void k_gn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, int *idx0, double *coef) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int gi = node[t];
@@ -474,11 +479,12 @@ __global__ void k_gn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int 
     int close_index = -1;
     bool inflag = false;
     for (int l = 0; l < 4; l++) {
-        const double s = sqrt(pow(x[l] - m.bi_x[gi], 2) + pow(y[l] - m.bi_y[gi], 2));
+        const double s = sqrt(sq(x[l] - m.bi_x[gi]) + sq(y[l] - m.bi_y[gi]));
         if (s < mn) { mn = s; close_index = index[l]; }
         if (m.ghostUV[index[l]] > 0) inflag = true;
     }
     for (int l = 0; l < 4; l++) {
+        idx0[4 * t + l] = index[l];
         if (m.ghostUV[index[l]] == gi) {
             x[l] = m.bi_x[index[l]];
             y[l] = m.bi_y[index[l]];
@@ -499,7 +505,8 @@ __global__ void k_gn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int 
 }
 
 // interpolateVelocityToHybridNodeX/Y weights and the luoIBM alpha (weight.cu:11-92)
-__global__ void k_hn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, double *coef, double *alpha) {
+__global__ void k_hn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, int *idx0, double *coef,
+                              double *alpha) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int h = node[t];
@@ -512,12 +519,14 @@ __global__ void k_hn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int 
     double y[4] = {c.ys[jj - 1], c.ys[jj - 1], c.ys[jj], c.ys[jj]};
     int index[4] = {(jj - 1) * c.stride + ii - 1 + c.off, (jj - 1) * c.stride + ii + c.off,
                     jj * c.stride + ii - 1 + c.off, jj * c.stride + ii + c.off};
-    for (int l = 0; l < 4; l++)
+    for (int l = 0; l < 4; l++) {
+        idx0[4 * t + l] = index[l];
         if (m.hybridUV[index[l]] == h) {
             x[l] = m.bi_x[h];
             y[l] = m.bi_y[h];
             index[l] = -1;
         }
+    }
     double a[16], w[4];
     bil_rows(x, y, a);
     bil_weights(a, c.xs[I], c.ys[J], w);
@@ -528,17 +537,17 @@ __global__ void k_hn_vel_coef(GridDev g, IbmGeom m, int n, const int *node, int 
     const int st = c.stride;
     double d1 = 0, d2 = 0;
     const double dxl = c.xs[I] - c.xs[I - 1], dyl = c.ys[J] - c.ys[J - 1];
-    if (m.ghostUV[h - 1] > 0) d1 = sqrt(pow(m.bi_x[h - 1] - c.xs[I - 1], 2) + pow((m.bi_y[h - 1] - c.ys[J]), 2));
-    else if (m.ghostUV[h + 1] > 0) d1 = sqrt(pow((m.bi_x[h + 1] - c.xs[I + 1]), 2) + pow((m.bi_y[h + 1] - c.ys[J]), 2));
-    if (m.ghostUV[h - st] > 0) d2 = sqrt(pow(m.bi_x[h - st] - c.xs[I], 2) + pow(m.bi_y[h - st] - c.ys[J - 1], 2));
-    if (m.ghostUV[h + st] > 0) d2 = sqrt(pow(m.bi_x[h + st] - c.xs[I], 2) + pow(m.bi_y[h + st] - c.ys[J + 1], 2));
-    alpha[t] = sqrt(pow(d1 / dxl, 2) + pow(d2 / dyl, 2));
+    if (m.ghostUV[h - 1] > 0) d1 = sqrt(sq(m.bi_x[h - 1] - c.xs[I - 1]) + sq((m.bi_y[h - 1] - c.ys[J])));
+    else if (m.ghostUV[h + 1] > 0) d1 = sqrt(sq((m.bi_x[h + 1] - c.xs[I + 1])) + sq((m.bi_y[h + 1] - c.ys[J])));
+    if (m.ghostUV[h - st] > 0) d2 = sqrt(sq(m.bi_x[h - st] - c.xs[I]) + sq(m.bi_y[h - st] - c.ys[J - 1]));
+    if (m.ghostUV[h + st] > 0) d2 = sqrt(sq(m.bi_x[h + st] - c.xs[I]) + sq(m.bi_y[h + st] - c.ys[J + 1]));
+    alpha[t] = sqrt(sq(d1 / dxl) + sq(d2 / dyl));
 }
 
 // interpolatePressureToHybridNode weights with the Neumann row (biLinearInterpolation.cu:671-856)
 // and weightP's alpha (luoIBM/kernels/weight.cu:95-134)
-__global__ void k_hn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, double *coef, double *nrm,
-                            double *alpha) {
+__global__ void k_hn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, int *idx0, double *coef,
+                            double *nrm, double *alpha) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int ip = node[t], nx = g.nx;
@@ -553,6 +562,7 @@ __global__ void k_hn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *i
     double a[16];
     bil_rows(x, y, a);
     for (int l = 0; l < 4; l++) {
+        idx0[4 * t + l] = index[l];
         nrm[8 * t + 2 * l] = 0;
         nrm[8 * t + 2 * l + 1] = 0;
         if (m.hybridP[index[l]] == ip) {
@@ -577,17 +587,17 @@ __global__ void k_hn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *i
     }
     double d1 = 0, d2 = 0;
     const double dxl = xv[I] - xv[I - 1], dyl = yu[J] - yu[J - 1];
-    if (m.ghostP[ip - 1] > 0) d1 = sqrt(pow(m.bip_x[ip - 1] - xv[I - 1], 2) + pow(m.bip_y[ip - 1] - yu[J], 2));
-    else if (m.ghostP[ip + 1] > 0) d1 = sqrt(pow(m.bip_x[ip + 1] - xv[I + 1], 2) + pow(m.bip_y[ip + 1] - yu[J], 2));
-    if (m.ghostP[ip - nx] > 0) d2 = sqrt(pow(m.bip_x[ip - nx] - xv[I], 2) + pow(m.bip_y[ip - nx] - yu[J - 1], 2));
-    if (m.ghostP[ip + nx] > 0) d2 = sqrt(pow(m.bip_x[ip + nx] - xv[I], 2) + pow(m.bip_y[ip + nx] - yu[J + 1], 2));
-    alpha[t] = sqrt(pow(d1 / dxl, 2) + pow(d2 / dyl, 2));
+    if (m.ghostP[ip - 1] > 0) d1 = sqrt(sq(m.bip_x[ip - 1] - xv[I - 1]) + sq(m.bip_y[ip - 1] - yu[J]));
+    else if (m.ghostP[ip + 1] > 0) d1 = sqrt(sq(m.bip_x[ip + 1] - xv[I + 1]) + sq(m.bip_y[ip + 1] - yu[J]));
+    if (m.ghostP[ip - nx] > 0) d2 = sqrt(sq(m.bip_x[ip - nx] - xv[I]) + sq(m.bip_y[ip - nx] - yu[J - 1]));
+    if (m.ghostP[ip + nx] > 0) d2 = sqrt(sq(m.bip_x[ip + nx] - xv[I]) + sq(m.bip_y[ip + nx] - yu[J + 1]));
+    alpha[t] = sqrt(sq(d1 / dxl) + sq(d2 / dyl));
 }
 
 // interpolatePressureToGhostNode weights (biLinearInterpolation.cu:859-1066); extra =
 // (distance image point - node, close-corner normal x, y)
-__global__ void k_gn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, double *coef, double *nrm,
-                            double *extra) {
+__global__ void k_gn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *idx, int *idx0, double *coef,
+                            double *nrm, double *extra) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int ip = node[t], nx = g.nx;
@@ -604,11 +614,12 @@ __global__ void k_gn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *i
     double mn = 1.0, cnx = 0, cny = 0;
     int close_index = -1;
     for (int l = 0; l < 4; l++) {
-        const double s = sqrt(pow(x[l] - m.bip_x[ip], 2) + pow(y[l] - m.bip_y[ip], 2));
+        const double s = sqrt(sq(x[l] - m.bip_x[ip]) + sq(y[l] - m.bip_y[ip]));
         if (s < mn) { mn = s; close_index = index[l]; }
     }
     int out_idx[4];
     for (int l = 0; l < 4; l++) {
+        idx0[4 * t + l] = index[l];
         out_idx[l] = index[l];
         nrm[8 * t + 2 * l] = 0;
         nrm[8 * t + 2 * l + 1] = 0;
@@ -640,7 +651,7 @@ __global__ void k_gn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *i
         idx[4 * t + l] = out_idx[l];
         coef[4 * t + l] = w[l];
     }
-    extra[3 * t] = sqrt(pow(m.ipp_x[ip] - xv[I], 2) + pow(m.ipp_y[ip] - yu[J], 2));
+    extra[3 * t] = sqrt(sq(m.ipp_x[ip] - xv[I]) + sq(m.ipp_y[ip] - yu[J]));
     extra[3 * t + 1] = cnx;
     extra[3 * t + 2] = cny;
 }
@@ -870,24 +881,28 @@ void launch_zero_rows(int *tags, const BoxDev &box, int rowlen, int kmax, int of
 
 void launch_gn_vel_coef(const GridDev &g, const IbmGeom &m, Stencils &st, hipStream_t s) {
     if (!st.n) return;
-    hipLaunchKernelGGL(k_gn_vel_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.coef.p());
+    hipLaunchKernelGGL(k_gn_vel_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.idx0.p(),
+                       st.coef.p());
     CUIBM_LAUNCH_CHECK();
 }
 void launch_hn_vel_coef(const GridDev &g, const IbmGeom &m, Stencils &st, hipStream_t s) {
     if (!st.n) return;
-    hipLaunchKernelGGL(k_hn_vel_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.coef.p(),
+    hipLaunchKernelGGL(k_hn_vel_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.idx0.p(),
+                       st.coef.p(),
                        st.extra.p());
     CUIBM_LAUNCH_CHECK();
 }
 void launch_hn_p_coef(const GridDev &g, const IbmGeom &m, Stencils &st, hipStream_t s) {
     if (!st.n) return;
-    hipLaunchKernelGGL(k_hn_p_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.coef.p(),
+    hipLaunchKernelGGL(k_hn_p_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.idx0.p(),
+                       st.coef.p(),
                        st.nrm.p(), st.extra.p());
     CUIBM_LAUNCH_CHECK();
 }
 void launch_gn_p_coef(const GridDev &g, const IbmGeom &m, Stencils &st, hipStream_t s) {
     if (!st.n) return;
-    hipLaunchKernelGGL(k_gn_p_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.coef.p(),
+    hipLaunchKernelGGL(k_gn_p_coef, dim3(nblk(st.n)), dim3(BS), 0, s, g, m, st.n, st.node.p(), st.idx.p(), st.idx0.p(),
+                       st.coef.p(),
                        st.nrm.p(), st.extra.p());
     CUIBM_LAUNCH_CHECK();
 }

@@ -332,7 +332,7 @@ long NavierStokesSolver::setField(const std::string &n, const void *in, long cnt
 }
 
 void NavierStokesSolver::info(double *out, int n) const {
-    std::vector<double> v(32, 0.0);
+    std::vector<double> v(40, 0.0);
     v[0] = timeStep;
     v[1] = iterationCount1;
     v[2] = iterationCount2;
@@ -350,7 +350,7 @@ void NavierStokesSolver::info(double *out, int n) const {
     v[29] = bytesTotal2;
     v[30] = solvesTotal2;
     v[31] = restartsTotal;
-    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 32));
+    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 40));
 }
 
 int NavierStokesSolver::phase(const std::string &n) {
@@ -636,6 +636,7 @@ void luo_base::buildStencils() {
         st.node.upload(list, s);
         if (list.empty()) st.node.alloc(1);
         st.idx.alloc(4 * m);
+        st.idx0.alloc(4 * m);
         st.coef.alloc(4 * m);
         st.nrm.alloc(8 * m);
         st.extra.alloc(nextra * m);
@@ -811,8 +812,8 @@ long luo_base::getField(const std::string &n, void *out, long cap) {
 
 void luo_base::info(double *out, int n) const {
     const_cast<luo_base *>(this)->readForces();
-    double v[32];
-    NavierStokesSolver::info(v, 32);
+    double v[40];
+    NavierStokesSolver::info(v, 40);
     v[5] = forceX;
     v[6] = forceY;
     v[7] = fxx;
@@ -829,7 +830,7 @@ void luo_base::info(double *out, int n) const {
     v[23] = gnP.n;
     v[24] = hnP.n;
     v[25] = nsolid;
-    std::memcpy(out, v, sizeof(double) * std::min(n, 32));
+    std::memcpy(out, v, sizeof(double) * std::min(n, 40));
 }
 
 int luo_base::phase(const std::string &n) {
@@ -923,6 +924,150 @@ int luoIBM::phase(const std::string &n) {
     return 0;
 }
 
+// ============================================================================ Luo embedded
+
+// luo_iter.cu:33-35 (luo_iter::cast is empty; the reference assembles both systems and
+// builds both preconditioners in every step's set-up, which for the geometry-only
+// matrices is reproduced by assembling once per tagging)
+void luoIter::initialise() {
+    luo_base::initialise();
+    initialiseLHS();
+}
+
+EmbAsm luoIter::asm1() const {
+    EmbAsm a{};
+    a.n = numUV;
+    a.map = map1.p();
+    a.hyb = EmbStencil{hnV.idx0.p(), hnV.idx.p(), hnV.coef.p(), hnV.nrm.p(), hnV.extra.p()};
+    a.gho = EmbStencil{gnV.idx0.p(), gnV.idx.p(), gnV.coef.p(), gnV.nrm.p(), gnV.extra.p()};
+    a.ghost = ghostTagsUV.p();
+    a.prp = LHS1.rp.p();
+    a.pci = LHS1.ci.p();
+    a.pv = LHS1.v.p();
+    a.cnt = cnt.p();
+    a.bc = rowc.p();
+    a.bv = rowv.p();
+    a.ns = ns1.p();
+    a.stats = stats.p();
+    return a;
+}
+
+EmbAsm luoIter::asm2() const {
+    EmbAsm a{};
+    a.n = numP;
+    a.map = map2.p();
+    a.hyb = EmbStencil{hnP.idx0.p(), hnP.idx.p(), hnP.coef.p(), hnP.nrm.p(), hnP.extra.p()};
+    a.gho = EmbStencil{gnP.idx0.p(), gnP.idx.p(), gnP.coef.p(), gnP.nrm.p(), gnP.extra.p()};
+    a.ghost = ghostTagsP.p();
+    a.prp = LHS2.rp.p();
+    a.pci = LHS2.ci.p();
+    a.pv = LHS2.v.p();
+    a.cnt = cnt.p();
+    a.bc = rowc.p();
+    a.bv = rowv.p();
+    a.ns = ns2.p();
+    a.stats = stats.p() + 4;
+    return a;
+}
+
+// row buffers -> CSR: the row counts come back to the host for the row pointers
+void luoIter::assemble(DCsr<double> &A, const EmbAsm &a, bool velocity, int statOff) {
+    if (velocity) launch_emb_lhs1(grid(), nu, dt, a, stream_);
+    else launch_emb_lhs2(grid(), dt, a, stream_);
+    std::vector<int> c(a.n);
+    cnt.download(c.data(), a.n, stream_);
+    std::vector<int> rp(a.n + 1, 0);
+    for (int i = 0; i < a.n; i++) rp[i + 1] = rp[i] + c[i];
+    DCsr<double> B;
+    B.n = a.n;
+    B.ncols = a.n;
+    B.nnz = rp.back();
+    B.rp.upload(rp, stream_);
+    B.ci.alloc(B.nnz);
+    B.v.alloc(B.nnz);
+    launch_emb_compact(a, B.rp.p(), B.ci.p(), B.v.p(), stream_);
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    A = std::move(B);
+    (void)statOff;
+}
+
+// intermediate_velocity_setup / poisson_setup, matrix half (luo_iter/intermediateVelocity.cu:
+// 80-176, luo_iter/poisson.cu:55-136): plain matrices for the boundary rows, row maps from
+// the node lists, the two embedded assemblies, then the preconditioners
+void luoIter::initialiseLHS() {
+    hipStream_t s = stream_;
+    generateLHS1();
+    generateLHS2();
+    // a node tagged both hybrid and ghost is a hybrid row: the reference's row kernels test
+    // the hybrid tag first, and its hybrid interpolation overwrites the ghost one's records
+    auto make_map = [&](DArray<int> &map, int n, const Stencils &hyb, const Stencils &gho) {
+        std::vector<int> m(n, -1);
+        if (gho.n) {
+            const std::vector<int> g = gho.node.to_host(s);
+            for (int t = 0; t < gho.n; t++) m[g[t]] = -(t + 2);
+        }
+        if (hyb.n) {
+            const std::vector<int> h = hyb.node.to_host(s);
+            for (int t = 0; t < hyb.n; t++) m[h[t]] = t;
+        }
+        map.upload(m, s);
+    };
+    make_map(map1, numUV, hnV, gnV);
+    make_map(map2, numP, hnP, gnP);
+    const int nmax = std::max(numUV, numP);
+    cnt.alloc(nmax);
+    rowc.alloc((size_t)EMB_ROW * nmax);
+    rowv.alloc((size_t)EMB_ROW * nmax);
+    ns1.alloc(numUV);
+    ns2.alloc(numP);
+    std::vector<int> zero(8, 0);
+    stats.upload(zero, s);
+    assemble(LHS1, asm1(), true, 0);
+    assemble(LHS2, asm2(), false, 4);
+    std::vector<int> st(8);
+    stats.download(st.data(), 8, s);
+    for (int k = 0; k < 8; k++) embStats[k] += st[k];
+    buildPreconditioners();
+}
+
+// luo_iter/intermediateVelocity.cu:17-77 (u = uold is the identity here) and the solve
+void luoIter::_intermediate_velocity() {
+    updateRobinBoundary();
+    launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_);
+    launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
+                opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
+    launch_emb_rhs1(rhs1.p(), u.p(), asm1(), grid(), bUB[0], bVB[0], stream_);
+    solveIntermediateVelocity();
+}
+
+// luo_iter/poisson.cu:15-52 and the solve
+void luoIter::_pressure() {
+    launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_);
+    const double dudt = (bUB[0] - uBk) / dt, dvdt = (bVB[0] - vBk) / dt;
+    launch_emb_rhs2(rhs2.p(), asm2(), grid(), dudt, dvdt, stream_);
+    solvePoisson();
+}
+
+// luo_base.cu:137-147 with the matrices and preconditioners of the moved body
+void luoIter::updateSolver() {
+    luo_base::updateSolver();
+    if (moving) initialiseLHS();
+}
+
+int luoIter::phase(const std::string &n) {
+    if (n == "assemble" || n == "initialiseLHS") initialiseLHS();
+    else return luo_base::phase(n);
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+void luoIter::info(double *out, int n) const {
+    double v[40];
+    luo_base::info(v, 40);
+    for (int k = 0; k < 8; k++) v[32 + k] = (double)embStats[k];
+    std::memcpy(out, v, sizeof(double) * std::min(n, 40));
+}
+
 // ============================================================================ modified Fadlun
 
 // fadlunModified.cu:28-59 (the matrices are assembled sorted, no sort needed)
@@ -1008,7 +1153,7 @@ std::unique_ptr<NavierStokesSolver> createSolver(ParameterDB *db, Domain *dom, c
         case NAVIERSTOKES: return std::unique_ptr<NavierStokesSolver>(new NavierStokesSolver(db, dom, opt));
         case FADLUN: return std::unique_ptr<NavierStokesSolver>(new fadlunModified(db, dom, opt));
         case LUO: return std::unique_ptr<NavierStokesSolver>(new luoIBM(db, dom, opt));
-        case OSC: throw Error("OSC_CYLINDER (Luo embedded) is not supported yet", ERR_UNSUPPORTED);
+        case OSC: return std::unique_ptr<NavierStokesSolver>(new luoIter(db, dom, opt));
         default: throw Error("solver type FSI is not implemented in the reference either", ERR_UNSUPPORTED);
     }
 }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "case_io.hpp"
+#include "embedded.hpp"
 #include "ibm.hpp"
 #include "kernels.hpp"
 #include "krylov.hpp"
@@ -122,7 +123,7 @@ class luo_base : public NavierStokesSolver {
     virtual void _update_body();
     virtual void _post_step();
     void moveBody();
-    void updateSolver();
+    virtual void updateSolver();
     void tagPoints();
     void calculateForce();
     void updateRobinBoundary();
@@ -183,6 +184,31 @@ class luoIBM : public luo_base {
 
   protected:
     bool centrePin() const override { return false; }
+};
+
+// Luo et al. embedded method (luo_iter.h): the interpolation lives in the matrices
+class luoIter : public luo_base {
+  public:
+    using luo_base::luo_base;
+    void initialise() override;
+    void initialiseLHS() override;
+    void _intermediate_velocity() override;
+    void _pressure() override;
+    void updateSolver() override;
+    std::string name() const override { return "OSC_CYLINDER"; }
+    int phase(const std::string &name) override;
+    void info(double *out, int n) const override;
+
+  protected:
+    bool centrePin() const override { return false; }
+    bool stencilLHS2() const override { return false; }  // interpolation rows
+    // row maps (-1 plain, t hybrid slot, -(t + 2) ghost slot), row buffers, row scales
+    DArray<int> map1, map2, cnt, rowc, stats;
+    DArray<double> rowv, ns1, ns2;
+    long embStats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    EmbAsm asm1() const;
+    EmbAsm asm2() const;
+    void assemble(DCsr<double> &A, const EmbAsm &a, bool velocity, int statOff);
 };
 
 // modified Fadlun direct forcing (fadlunModified.h)

@@ -122,7 +122,11 @@ long cuibm_set_field(cuibm_solver *s, const char *name, const void *host, long n
  * [21..25] ghost/hybrid velocity, ghost/hybrid pressure and solid node counts,
  * running totals since initialise: [26..27] velocity/Poisson iterations,
  * [28..29] velocity/Poisson solve bytes, [30] Poisson solves, [31] BiCGSTAB
- * breakdown restarts */
+ * breakdown restarts; OSC_CYLINDER only, summed over all assemblies of LHS1 then
+ * LHS2: [32|36] 5-point slots a ghost row leaves unwritten, [33|37] hybrid tail slots
+ * never written, [34|38] tail slots written twice, [35|39] interpolation corners outside
+ * the 3x3 stencil (the reference's result depends on stale slots when any is nonzero).
+ * n may be up to 40. */
 int cuibm_info(cuibm_solver *s, double *out, int n);
 
 /* algorithmic HBM bytes of the last step: which = 0 whole step, 1 velocity solve,

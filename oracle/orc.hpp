@@ -128,6 +128,19 @@ struct Solver {
     std::vector<double> ustar, pstar, image_point_u, dpdn;
     std::vector<double> qcoef[4];
     std::vector<int> qidx[4];
+    // corner values of the last velocity interpolation (body-intercept corners hold uB)
+    std::vector<double> qval[4];
+    // pressure interpolation per node: corner indices, weights at the node (hybrid) or
+    // image point (ghost), corner values (Neumann corners hold their normal-derivative value)
+    std::vector<double> pqcoef[4], pqval[4];
+    std::vector<int> pqidx[4];
+    // Luo embedded (luo_iter): blending weights and the row scale / added right-hand side
+    std::vector<double> alpha1, alpha2, ns_rhs1, interp_rhs1, ns_rhs2, interp_rhs2;
+    // slot accounting of the embedded assembly, summed over all steps:
+    // [0] 5-point slots a ghost row leaves unwritten, [1] hybrid tail slots never written,
+    // [2] hybrid tail slots written more than once (last write kept), [3] interpolation
+    // corners outside the 3x3 stencil (dropped); LHS1 in [0..3], LHS2 in [4..7]
+    long emb_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool has_body = false;
 
     void init(const Params &P, const Grid &G, const std::vector<double> &bx,
@@ -164,6 +177,14 @@ struct Solver {
     void step_luo();
     void move_body();
     void step_fadlun();
+    // Luo embedded
+    void step_embedded();
+    void alpha_uv();
+    void alpha_pressure();
+    void buildLHS1_embedded();
+    void buildLHS2_embedded();
+    void update_rhs1_embedded();
+    void update_rhs2_embedded();
     std::vector<double> *field(const std::string &name);
     std::vector<int> *ifield(const std::string &name);
 };

@@ -29,17 +29,21 @@ namespace orc {
 
 static const double EPS = 1.e-10;
 
+// squares are products (pow(x, 2) in the reference): the same rounding on every platform
+static inline double sq(double x) { return x * x; }
+
 // foot of the perpendicular from (px,py) to the line through body points k,l
 // (tagPoints.cu:122-133, same construction at every call site)
 static inline void foot(const double *bx, const double *by, int k, int l, double px, double py,
                         double &fx, double &fy) {
-    const double p = std::sqrt(std::pow((bx[l] - bx[k]), 2) + std::pow((by[l] - by[k]), 2));
-    const double o = std::sqrt(std::pow((bx[l] - px), 2) + std::pow((by[l] - py), 2));
-    const double b = std::sqrt(std::pow((px - bx[k]), 2) + std::pow((py - by[k]), 2));
-    const double th = std::acos((p * p + b * b - o * o) / (2 * p * b));
-    const double a = std::sin(th) * b;
-    fy = by[k] + (by[l] - by[k]) / p * a / std::tan(th);
-    fx = bx[k] + (bx[l] - bx[k]) / p * a / std::tan(th);
+    // a / tan(theta) of the reference, with theta = acos(c), a = b sin(theta) and
+    // c = (p^2 + b^2 - o^2) / (2 p b), is b cos(theta) = (p^2 + b^2 - o^2) / (2 p)
+    const double p = std::sqrt(sq((bx[l] - bx[k])) + sq((by[l] - by[k])));
+    const double o = std::sqrt(sq((bx[l] - px)) + sq((by[l] - py)));
+    const double b = std::sqrt(sq((px - bx[k])) + sq((py - by[k])));
+    const double d = (p * p + b * b - o * o) / (2 * p);
+    fy = by[k] + (by[l] - by[k]) / p * d;
+    fx = bx[k] + (bx[l] - bx[k]) / p * d;
 }
 
 // src/bodies.cu:257-326 (the box is computed from the body points; x,y are cell centres)
@@ -471,7 +475,7 @@ void Solver::gn_velocity(std::vector<double> &f, bool set) {
                 int close_index = -1;
                 bool inflag = false;
                 for (int l = 0; l < 4; l++) {
-                    const double s = std::sqrt(std::pow(x[l] - bi_x[g], 2) + std::pow(y[l] - bi_y[g], 2));
+                    const double s = std::sqrt(sq(x[l] - bi_x[g]) + sq(y[l] - bi_y[g]));
                     if (s < mn) { mn = s; close_index = index[l]; }
                     if (ghostTagsUV[index[l]] > 0) inflag = true;
                 }
@@ -497,6 +501,7 @@ void Solver::gn_velocity(std::vector<double> &f, bool set) {
                 for (int l = 0; l < 4; l++) {
                     qcoef[l][g] = (B4.b[0][l] + B4.b[1][l] * X + B4.b[2][l] * Y + B4.b[3][l] * X * Y) / d;
                     qidx[l][g] = index[l];
+                    qval[l][g] = q[l];
                 }
                 image_point_u[g] = av[0] + av[1] * X + av[2] * Y + av[3] * X * Y;
                 if (set) out.push_back({g, 2 * ubody[c] - image_point_u[g]});
@@ -553,6 +558,7 @@ void Solver::hn_velocity(const std::vector<double> &f) {
                 for (int l = 0; l < 4; l++) {
                     qcoef[l][h] = (B4.b[0][l] + B4.b[1][l] * X + B4.b[2][l] * Y + B4.b[3][l] * X * Y) / d;
                     qidx[l][h] = index[l];
+                    qval[l][h] = q[l];
                 }
                 ustar[h] = av[0] + av[1] * X + av[2] * Y + av[3] * Y * X;
                 image_point_u[h] = av[0] + av[1] * ip_x[h] + av[2] * ip_y[h] + av[3] * ip_x[h] * ip_y[h];
@@ -575,14 +581,14 @@ void Solver::weight_uhat() {
                 double d1 = 0, d2 = 0;
                 const double dxl = xs[I] - xs[I - 1], dyl = ys[J] - ys[J - 1];
                 if (ghostTagsUV[h - 1] > 0)
-                    d1 = std::sqrt(std::pow(bi_x[h - 1] - xs[I - 1], 2) + std::pow((bi_y[h - 1] - ys[J]), 2));
+                    d1 = std::sqrt(sq(bi_x[h - 1] - xs[I - 1]) + sq((bi_y[h - 1] - ys[J])));
                 else if (ghostTagsUV[h + 1] > 0)
-                    d1 = std::sqrt(std::pow((bi_x[h + 1] - xs[I + 1]), 2) + std::pow((bi_y[h + 1] - ys[J]), 2));
+                    d1 = std::sqrt(sq((bi_x[h + 1] - xs[I + 1])) + sq((bi_y[h + 1] - ys[J])));
                 if (ghostTagsUV[h - st] > 0)
-                    d2 = std::sqrt(std::pow(bi_x[h - st] - xs[I], 2) + std::pow(bi_y[h - st] - ys[J - 1], 2));
+                    d2 = std::sqrt(sq(bi_x[h - st] - xs[I]) + sq(bi_y[h - st] - ys[J - 1]));
                 if (ghostTagsUV[h + st] > 0)
-                    d2 = std::sqrt(std::pow(bi_x[h + st] - xs[I], 2) + std::pow(bi_y[h + st] - ys[J + 1], 2));
-                const double alpha = std::sqrt(std::pow(d1 / dxl, 2) + std::pow(d2 / dyl, 2));
+                    d2 = std::sqrt(sq(bi_x[h + st] - xs[I]) + sq(bi_y[h + st] - ys[J + 1]));
+                const double alpha = std::sqrt(sq(d1 / dxl) + sq(d2 / dyl));
                 uhat[h] = (1 - alpha) * uhat[h] + alpha * ustar[h];
             }
     }
@@ -625,6 +631,12 @@ void Solver::hn_pressure() {
             for (int r = 0; r < 4; r++)
                 av[r] = B4.b[r][0] / d * q[0] + B4.b[r][1] / d * q[1] + B4.b[r][2] / d * q[2] + B4.b[r][3] / d * q[3];
             pstar[ip] = av[0] + av[1] * xv[I] + av[2] * yu[J] + av[3] * xv[I] * yu[J];
+            // biLinearInterpolation.cu:775-785, 850-853
+            for (int l = 0; l < 4; l++) {
+                pqcoef[l][ip] = (B4.b[0][l] + B4.b[1][l] * xv[I] + B4.b[2][l] * yu[J] + B4.b[3][l] * xv[I] * yu[J]) / d;
+                pqidx[l][ip] = index[l];
+                pqval[l][ip] = q[l];
+            }
         }
 }
 
@@ -638,14 +650,14 @@ void Solver::weight_p() {
             double d1 = 0, d2 = 0;
             const double dxl = xv[I] - xv[I - 1], dyl = yu[J] - yu[J - 1];
             if (ghostTagsP[ip - 1] > 0)
-                d1 = std::sqrt(std::pow(bip_x[ip - 1] - xv[I - 1], 2) + std::pow(bip_y[ip - 1] - yu[J], 2));
+                d1 = std::sqrt(sq(bip_x[ip - 1] - xv[I - 1]) + sq(bip_y[ip - 1] - yu[J]));
             else if (ghostTagsP[ip + 1] > 0)
-                d1 = std::sqrt(std::pow(bip_x[ip + 1] - xv[I + 1], 2) + std::pow(bip_y[ip + 1] - yu[J], 2));
+                d1 = std::sqrt(sq(bip_x[ip + 1] - xv[I + 1]) + sq(bip_y[ip + 1] - yu[J]));
             if (ghostTagsP[ip - nx] > 0)
-                d2 = std::sqrt(std::pow(bip_x[ip - nx] - xv[I], 2) + std::pow(bip_y[ip - nx] - yu[J - 1], 2));
+                d2 = std::sqrt(sq(bip_x[ip - nx] - xv[I]) + sq(bip_y[ip - nx] - yu[J - 1]));
             if (ghostTagsP[ip + nx] > 0)
-                d2 = std::sqrt(std::pow(bip_x[ip + nx] - xv[I], 2) + std::pow(bip_y[ip + nx] - yu[J + 1], 2));
-            const double alpha = std::sqrt(std::pow(d1 / dxl, 2) + std::pow(d2 / dyl, 2));
+                d2 = std::sqrt(sq(bip_x[ip + nx] - xv[I]) + sq(bip_y[ip + nx] - yu[J + 1]));
+            const double alpha = std::sqrt(sq(d1 / dxl) + sq(d2 / dyl));
             p[ip] = (1 - alpha) * p[ip] + alpha * pstar[ip];
         }
 }
@@ -670,7 +682,7 @@ void Solver::gn_pressure(bool set) {
             double mn = 1.0, matDClose = 0;
             int close_index = -1;
             for (int l = 0; l < 4; l++) {
-                const double s = std::sqrt(std::pow(x[l] - bip_x[ip], 2) + std::pow(y[l] - bip_y[ip], 2));
+                const double s = std::sqrt(sq(x[l] - bip_x[ip]) + sq(y[l] - bip_y[ip]));
                 if (s < mn) { mn = s; close_index = index[l]; }
             }
             for (int l = 0; l < 4; l++) {
@@ -701,7 +713,13 @@ void Solver::gn_pressure(bool set) {
                 av[r] = B4.b[r][0] / d * q[0] + B4.b[r][1] / d * q[1] + B4.b[r][2] / d * q[2] + B4.b[r][3] / d * q[3];
             const double X = ipp_x[ip], Y = ipp_y[ip];
             const double ipp = av[0] + av[1] * X + av[2] * Y + av[3] * Y * X;
-            dpdn[ip] = std::sqrt(std::pow(X - xv[I], 2) + std::pow(Y - yu[J], 2)) * matDClose;
+            dpdn[ip] = std::sqrt(sq(X - xv[I]) + sq(Y - yu[J])) * matDClose;
+            // biLinearInterpolation.cu:972-982, 1053-1056
+            for (int l = 0; l < 4; l++) {
+                pqcoef[l][ip] = (B4.b[0][l] + B4.b[1][l] * X + B4.b[2][l] * Y + B4.b[3][l] * X * Y) / d;
+                pqidx[l][ip] = index[l];
+                pqval[l][ip] = q[l];
+            }
             if (set) p[ip] = ipp + dpdn[ip];
         }
 }

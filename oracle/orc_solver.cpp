@@ -63,14 +63,23 @@ void Solver::init(const Params &Pin, const Grid &Gin, const std::vector<double> 
     bc[XMINUS][ny - 1] = P.bc_u[XMINUS];
     bc[XPLUS][ny - 1] = P.bc_u[XPLUS];
 
-    has_body = (P.solver_type == LUO || P.solver_type == FADLUN) && !bx.empty();
+    has_body = (P.solver_type == LUO || P.solver_type == FADLUN || P.solver_type == OSC) && !bx.empty();
     if (has_body) {
         for (auto *v : {&ghostTagsUV, &hybridTagsUV, &hybridTagsUV2}) v->assign(numUV, -1);
         for (auto *v : {&ghostTagsP, &hybridTagsP}) v->assign(numP, -1);
         for (auto *v : {&bi_x, &bi_y, &ip_x, &ip_y, &uv}) v->assign(numUV, 0.0);
         for (auto *v : {&dist_int, &dist_between}) v->assign(numUV, 1.0);
         for (auto *v : {&bip_x, &bip_y, &ipp_x, &ipp_y, &dist_u_body, &dist_v_body}) v->assign(numP, 0.0);
-        for (int l = 0; l < 4; l++) { qcoef[l].assign(numUV, 0.0); qidx[l].assign(numUV, 0); }
+        for (int l = 0; l < 4; l++) {
+            qcoef[l].assign(numUV, 0.0);
+            qidx[l].assign(numUV, 0);
+            qval[l].assign(numUV, 0.0);
+            pqcoef[l].assign(numP, 0.0);
+            pqidx[l].assign(numP, 0);
+            pqval[l].assign(numP, 0.0);
+        }
+        for (auto *v : {&alpha1, &ns_rhs1, &interp_rhs1}) v->assign(numUV, 0.0);
+        for (auto *v : {&alpha2, &ns_rhs2, &interp_rhs2}) v->assign(numP, 0.0);
         // bodies.cu:20-185: points, bounding box from the un-shifted points, centroid
         B.n = (int)bx.size();
         B.x = bx;
@@ -100,6 +109,9 @@ void Solver::init(const Params &Pin, const Grid &Gin, const std::vector<double> 
         tagPoints();
         if (P.solver_type == LUO) set_inside(u);  // luoIBM.cu:42
     }
+    // luo_base::initialiseLHS is empty: the embedded method assembles both systems and
+    // builds both preconditioners inside every step (luo_iter/intermediateVelocity.cu:64-71)
+    if (P.solver_type == OSC && has_body) return;
     buildLHS1();
     buildLHS2();
     PC1.build(LHS1, P.precond1);
@@ -207,6 +219,7 @@ void Solver::step_fadlun() {
 
 void Solver::step() {
     if (P.solver_type == LUO && has_body) step_luo();
+    else if (P.solver_type == OSC && has_body) step_embedded();
     else if (P.solver_type == FADLUN && has_body) step_fadlun();
     else step_ns();
 }
@@ -251,10 +264,30 @@ std::vector<double> *Solver::field(const std::string &n) {
     if (n == "body_y") return &B.y;
     if (n == "LHS1_val") return &LHS1.v;
     if (n == "LHS2_val") return &LHS2.v;
+    // interpolation records and the embedded method's per-row data
+    if (n == "alpha1") return &alpha1;
+    if (n == "alpha2") return &alpha2;
+    if (n == "ns_rhs1") return &ns_rhs1;
+    if (n == "interp_rhs1") return &interp_rhs1;
+    if (n == "ns_rhs2") return &ns_rhs2;
+    if (n == "interp_rhs2") return &interp_rhs2;
+    if (n == "dpdn") return &dpdn;
+    for (int l = 0; l < 4; l++) {
+        const std::string k = std::to_string(l);
+        if (n == "qcoef" + k) return &qcoef[l];
+        if (n == "qval" + k) return &qval[l];
+        if (n == "pqcoef" + k) return &pqcoef[l];
+        if (n == "pqval" + k) return &pqval[l];
+    }
     return nullptr;
 }
 
 std::vector<int> *Solver::ifield(const std::string &n) {
+    for (int l = 0; l < 4; l++) {
+        const std::string k = std::to_string(l);
+        if (n == "qidx" + k) return &qidx[l];
+        if (n == "pqidx" + k) return &pqidx[l];
+    }
     if (n == "ghostTagsUV") return &ghostTagsUV;
     if (n == "hybridTagsUV") return &hybridTagsUV;
     if (n == "hybridTagsUV2") return &hybridTagsUV2;

@@ -83,6 +83,8 @@ def lib():
         L.orc_amg_coarse_dense.argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
         L.orc_amg_prolongator_dense.restype = C.c_int
         L.orc_amg_prolongator_dense.argtypes = [C.c_void_p, _DP, C.c_int]
+        L.orc_embedded_stats.restype = None
+        L.orc_embedded_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -239,7 +241,8 @@ class Oracle:
             raise ValueError(name)
 
     _INT = {"ghostTagsUV", "hybridTagsUV", "hybridTagsUV2", "ghostTagsP", "hybridTagsP",
-            "LHS1_rowptr", "LHS1_col", "LHS2_rowptr", "LHS2_col"}
+            "LHS1_rowptr", "LHS1_col", "LHS2_rowptr", "LHS2_col",
+            "qidx0", "qidx1", "qidx2", "qidx3", "pqidx0", "pqidx1", "pqidx2", "pqidx3"}
 
     def get(self, name):
         n = lib().orc_get(self.h, name.encode(), None, 0)
@@ -260,6 +263,15 @@ class Oracle:
         keys = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "fxx", "fxy", "fxu",
                 "startI", "startJ", "numCellsX", "numCellsY", "midX", "midY", "amg_levels"]
         return dict(zip(keys, out[:len(keys)]))
+
+    def embedded_stats(self):
+        """Slot anomalies of the embedded assembly summed over all steps: for LHS1 then
+        LHS2, 5-point slots left unwritten by ghost rows, hybrid tail slots never written,
+        tail slots written more than once, interpolation corners outside the 3x3 stencil."""
+        out = (C.c_long * 8)()
+        lib().orc_embedded_stats(self.h, out)
+        keys = ["unwritten", "tail_unwritten", "tail_multi", "dropped"]
+        return {"LHS1": dict(zip(keys, out[0:4])), "LHS2": dict(zip(keys, out[4:8]))}
 
 
 class OrcCsr:

@@ -1,0 +1,90 @@
+"""Luo embedded method (SolverType OSC_CYLINDER) on the GPU against the oracle.
+
+Bars: the assembled LHS1 and LHS2 (interpolation folded into the rows) have the oracle's
+row pointers and columns exactly and values within 1e-15 relative L2 (the intercepts,
+weights and blending factors use only correctly rounded operations on both sides, so in
+practice they agree bit for bit; a corner set decided by an image point lying on a grid
+line would differ if they did not); after each step u
+is within 1e-10 relative L2, p within 1e-10 with its mean removed (the Poisson matrix is
+pinned only by row 0's doubled diagonal, so its constant mode is fixed weakly) and its
+mean within 1e-8, drag and lift within 1e-8; velocity iteration counts agree, Poisson
+counts within 3. The reference's
+fixed-slot assembly must leave no slot unwritten or written twice on these cases (if it
+did, its matrix would depend on the previous step's sorted arrays).
+"""
+import numpy as np
+import pytest
+
+from helpers import pair, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TAGS = ("ghostTagsUV", "hybridTagsUV", "ghostTagsP", "hybridTagsP")
+EMB_KEYS = ("emb1_unwritten", "emb1_tail_unwritten", "emb1_tail_multi", "emb1_dropped",
+            "emb2_unwritten", "emb2_tail_unwritten", "emb2_tail_multi", "emb2_dropped")
+
+
+def _compare_step(g, o, k):
+    du = rel_l2(g.get("u"), o.get("u"))
+    pg, po_ = g.get("p"), o.get("p")
+    dp = rel_l2(pg - pg.mean(), po_ - po_.mean())
+    dm = abs(pg.mean() - po_.mean()) / max(np.sqrt(np.mean(po_ ** 2)), 1e-300)
+    gi, oi = g.info(), o.info()
+    print(f"step {k}: u {du:.2e} p-mean {dp:.2e} mean {dm:.2e} it {gi['iter1']:.0f}/{oi['iter1']:.0f} "
+          f"{gi['iter2']:.0f}/{oi['iter2']:.0f} Fx {gi['forceX']:.12g} {oi['forceX']:.12g}", flush=True)
+    assert du < 1e-10, (k, du)
+    assert dp < 1e-10, (k, dp)
+    assert dm < 1e-8, (k, dm)
+    for f in ("forceX", "forceY"):
+        assert abs(gi[f] - oi[f]) <= 1e-8 * max(1.0, abs(oi[f])), (k, f, gi[f], oi[f])
+    assert gi["iter1"] == oi["iter1"], k
+    # at a 1e-12..1e-13 stop the last Poisson iterations sit at the rounding floor, so
+    # where BiCGSTAB stops can move by a few iterations between the two reduction orders
+    assert abs(gi["iter2"] - oi["iter2"]) <= 3, k
+
+
+def _no_anomalies(g, o):
+    gi = g.info()
+    assert all(gi[k] == 0 for k in EMB_KEYS), {k: gi[k] for k in EMB_KEYS}
+    st = o.embedded_stats()
+    assert all(v == 0 for d in st.values() for v in d.values()), st
+
+
+@pytest.mark.parametrize("name", ["cylinderRe40_small_embedded", "cylinderRe40_embedded"])
+def test_embedded_matrices_match_oracle(name):
+    g, o = pair(name, {"velocityTol": 1e-12, "poissonTol": 1e-12})
+    o.step(1)  # the oracle assembles inside the step, from the same tags
+    for m in ("LHS1", "LHS2"):
+        assert np.array_equal(g.get(m + "_rowptr"), o.get(m + "_rowptr")), m
+        assert np.array_equal(g.get(m + "_col"), o.get(m + "_col")), m
+        gv, ov = g.get(m + "_val"), o.get(m + "_val")
+        diff = np.nonzero(gv != ov)[0]
+        print(m, "entries differing:", len(diff), "of", len(ov), "max rel",
+              float(np.max(np.abs(gv - ov)[diff] / np.abs(ov)[diff])) if len(diff) else 0.0, flush=True)
+        assert rel_l2(gv, ov) < 1e-15, m
+    _no_anomalies(g, o)
+
+
+@pytest.mark.parametrize("name,steps", [("cylinderRe40_small_embedded", 10), ("cylinderRe40_embedded", 5)])
+def test_embedded_steps_match_oracle(name, steps):
+    g, o = pair(name, {"velocityTol": 1e-12, "poissonTol": 1e-12})
+    for k in range(steps):
+        g.step_time()
+        o.step()
+        _compare_step(g, o, k)
+    _no_anomalies(g, o)
+
+
+def test_embedded_oscillating_cylinder_matches_oracle():
+    """In-line oscillation with the embedded method (validation/error/oscflow2/embedded05
+    of the reference): after each step's move the tags, both matrices and both
+    preconditioners are rebuilt."""
+    g, o = pair("oscFlowRe100_embedded", {"velocityTol": 1e-12, "poissonTol": 1e-13})
+    for k in range(6):
+        g.step_time()
+        o.step()
+        for t in TAGS:
+            assert np.array_equal(g.get(t), o.get(t)), (k, t)
+        assert g.info()["midX"] == o.info()["midX"], k
+        _compare_step(g, o, k)
+    _no_anomalies(g, o)

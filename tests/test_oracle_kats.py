@@ -413,3 +413,188 @@ def test_oscillating_body_motion_matches_reference_run():
     # an off-by-one in the motion's time would shift x by dt * u = 1e-3 per step
     assert abs(mid[5] - ref["midX"][4]) > 1e-4
     assert its == ref["iter_velocity"][:n]
+
+
+# ---- Luo embedded method (OSC_CYLINDER) -----------------------------------------------
+# The embedded rows fold the ghost/hybrid interpolation into the matrices. What pins them
+# independently of the assembly formulas: the interpolation weights reproduce bilinear
+# fields (Neumann corners included), and constant fields are preserved row by row.
+
+def _embedded(name="cylinderRe40_small_embedded", steps=1):
+    o = po.Oracle(po.read_case(case(name)))
+    o.step(steps)
+    st = o.embedded_stats()
+    assert all(v == 0 for d in st.values() for v in d.values()), st
+    return o
+
+
+def _emb_csr(o, which):
+    rp, ci, v = o.get(f"LHS{which}_rowptr"), o.get(f"LHS{which}_col"), o.get(f"LHS{which}_val")
+    return sp.csr_matrix((v, ci, rp), shape=(len(rp) - 1, len(rp) - 1))
+
+
+def _grid(o):
+    return {k: o.get(k) for k in ("x", "y", "xu", "yv")}
+
+
+def _vel_pos(g, i, nx, ny):
+    numU = (nx - 1) * ny
+    if i < numU:
+        return g["xu"][i % (nx - 1)], g["y"][i // (nx - 1)]
+    k = i - numU
+    return g["x"][k % nx], g["yv"][k // nx]
+
+
+def test_embedded_velocity_weights_reproduce_bilinear_fields():
+    """Ghost nodes: the corner equal to the node, or else (no ghost corner at all) the
+    corner nearest the body intercept, sits at the intercept (biLinearInterpolation.cu:
+    60-110); the weights evaluate at the image point. Hybrid nodes: the corner equal to
+    the node sits at its intercept; the weights evaluate at the node."""
+    o = _embedded()
+    nx, ny = len(o.get("x")), len(o.get("y"))
+    g = _grid(o)
+    gt, ht = o.get("ghostTagsUV"), o.get("hybridTagsUV")
+    bix, biy, ipx, ipy = o.get("body_intercept_x"), o.get("body_intercept_y"), o.get("image_point_x"), o.get("image_point_y")
+    C = np.stack([o.get(f"qcoef{l}") for l in range(4)])
+    Q = np.stack([o.get(f"qidx{l}") for l in range(4)])
+    rng = np.random.default_rng(7)
+    a, b, c, d = rng.standard_normal(4)
+    f = lambda x, y: a + b * x + c * y + d * x * y
+    nodes = np.nonzero((gt > 0) | (ht > 0))[0]
+    assert len(nodes) > 20
+    worst = 0.0
+    for i in nodes:
+        pos = [_vel_pos(g, int(Q[l, i]), nx, ny) for l in range(4)]
+        if gt[i] > 0:
+            inflag = any(gt[Q[l, i]] > 0 for l in range(4))
+            mn, close = 1.0, -1
+            for l in range(4):
+                s = np.hypot(pos[l][0] - bix[i], pos[l][1] - biy[i])
+                if s < mn:
+                    mn, close = s, Q[l, i]
+            for l in range(4):
+                if Q[l, i] == i or (Q[l, i] == close and not inflag):
+                    pos[l] = (bix[i], biy[i])
+            target = f(ipx[i], ipy[i])
+        else:
+            for l in range(4):
+                if Q[l, i] == i:
+                    pos[l] = (bix[i], biy[i])
+            target = f(*_vel_pos(g, i, nx, ny))
+        val = sum(C[l, i] * f(*pos[l]) for l in range(4))
+        worst = max(worst, abs(val - target) / (1 + abs(target)))
+    assert worst < 1e-9, worst
+
+
+def test_embedded_pressure_weights_reproduce_bilinear_fields_with_neumann_corners():
+    """Replaced pressure corners carry the normal derivative along (image point - body
+    intercept) of that corner (biLinearInterpolation.cu:740-770, 930-970): a bilinear
+    field f gives the value grad f . n there, and the weights must return f at the node
+    (hybrid) or at the image point (ghost)."""
+    o = _embedded()
+    x, y = o.get("x"), o.get("y")
+    nx = len(x)
+    gt, ht = o.get("ghostTagsP"), o.get("hybridTagsP")
+    bx, by = o.get("body_intercept_p_x"), o.get("body_intercept_p_y")
+    ix, iy = o.get("image_point_p_x"), o.get("image_point_p_y")
+    C = np.stack([o.get(f"pqcoef{l}") for l in range(4)])
+    Q = np.stack([o.get(f"pqidx{l}") for l in range(4)])
+    rng = np.random.default_rng(8)
+    a, b, c, d = rng.standard_normal(4)
+    f = lambda X, Y: a + b * X + c * Y + d * X * Y
+    grad = lambda X, Y: (b + d * Y, c + d * X)
+    nodes = np.nonzero((gt > 0) | (ht > 0))[0]
+    assert len(nodes) > 10
+    worst = 0.0
+    for i in nodes:
+        val = 0.0
+        for l in range(4):
+            j = int(Q[l, i])
+            neumann = (gt[j] > 0) if gt[i] > 0 else (j == i)
+            if neumann:
+                n = np.array([ix[j] - bx[j], iy[j] - by[j]])
+                n /= np.linalg.norm(n)
+                gx, gy = grad(bx[j], by[j])
+                val += C[l, i] * (gx * n[0] + gy * n[1])
+            else:
+                val += C[l, i] * f(x[j % nx], y[j // nx])
+        target = f(ix[i], iy[i]) if gt[i] > 0 else f(x[i % nx], y[i // nx])
+        worst = max(worst, abs(val - target) / (1 + abs(target)))
+    assert worst < 1e-9, worst
+
+
+def test_embedded_rows_preserve_constants():
+    """A constant field with the body moving at that constant is a solution of every
+    row: interior velocity rows sum to ns_rhs + alpha * (weight of the node's own corner),
+    ghost velocity rows sum to 2 - (weights of the corners kept explicit), interior and
+    wall pressure rows sum to 0 (row 0 carries the doubled-diagonal pin). A dropped or
+    sign-flipped interpolation entry breaks one of these."""
+    o = _embedded()
+    x, y = o.get("x"), o.get("y")
+    nx, ny = len(x), len(y)
+    numU = (nx - 1) * ny
+    A1, A2 = _emb_csr(o, 1), _emb_csr(o, 2)
+    s1, s2 = np.asarray(A1.sum(axis=1)).ravel(), np.asarray(A2.sum(axis=1)).ravel()
+    gt, ht = o.get("ghostTagsUV"), o.get("hybridTagsUV")
+    ns, al = o.get("ns_rhs1"), o.get("alpha1")
+    C = np.stack([o.get(f"qcoef{l}") for l in range(4)])
+    Q = np.stack([o.get(f"qidx{l}") for l in range(4)])
+    checked = {"hybrid": 0, "ghost": 0, "plain": 0}
+    for i in range(len(s1)):
+        if i < numU:
+            I, J, ni, nj = i % (nx - 1), i // (nx - 1), nx - 1, ny
+        else:
+            I, J, ni, nj = (i - numU) % nx, (i - numU) // nx, nx, ny - 1
+        if I == 0 or I == ni - 1 or J == 0 or J == nj - 1:
+            continue
+        if ht[i] > 0:
+            own = sum(C[l, i] for l in range(4) if Q[l, i] == i)
+            expect = ns[i] + al[i] * own
+            checked["hybrid"] += 1
+        elif gt[i] > 0:
+            expect = 2 - sum(C[l, i] for l in range(4) if gt[Q[l, i]] > 0 or Q[l, i] == i)
+            checked["ghost"] += 1
+        else:
+            expect = ns[i]
+            checked["plain"] += 1
+        assert abs(s1[i] - expect) < 1e-12 * (1 + abs(expect)), (i, s1[i], expect)
+    assert min(checked.values()) > 10, checked
+    scale = np.asarray(abs(A2).sum(axis=1)).ravel()
+    assert np.all(np.abs(s2[1:]) <= 1e-12 * scale[1:])
+    assert s2[0] > 0
+
+
+def test_embedded_plain_rows_are_the_navier_stokes_rows_over_their_diagonal():
+    """Velocity rows away from the body are the plain LHS1 rows (pinned above) divided by
+    their diagonal (LHS1.cu:162-207); pressure rows away from the body are the plain LHS2
+    rows (LHS2.cu:171-222)."""
+    o = _embedded()
+    c = po.read_case(case("cylinderRe40_small_embedded"))
+    c["sim"]["SolverType"] = "LUO"
+    p = po.Oracle(c)
+    E1, P1 = _emb_csr(o, 1), _emb_csr(p, 1)
+    E2, P2 = _emb_csr(o, 2), _emb_csr(p, 2)
+    gt, ht = o.get("ghostTagsUV"), o.get("hybridTagsUV")
+    x, y = o.get("x"), o.get("y")
+    nx, ny = len(x), len(y)
+    numU = (nx - 1) * ny
+    n_checked = 0
+    for i in range(0, E1.shape[0], 7):
+        if gt[i] != -1 or ht[i] != -1:
+            continue
+        if i < numU:
+            I, J, ni, nj = i % (nx - 1), i // (nx - 1), nx - 1, ny
+        else:
+            I, J, ni, nj = (i - numU) % nx, (i - numU) // nx, nx, ny - 1
+        er, pr = E1.getrow(i), P1.getrow(i)
+        assert list(er.indices) == list(pr.indices)
+        if I == 0 or I == ni - 1 or J == 0 or J == nj - 1:
+            assert np.array_equal(er.data, pr.data)
+        else:
+            np.testing.assert_allclose(er.data, pr.data / pr[0, i], rtol=1e-14, atol=0)
+        n_checked += 1
+    assert n_checked > 100
+    gp, hp = o.get("ghostTagsP"), o.get("hybridTagsP")
+    far = np.nonzero((gp == -1) & (hp == -1))[0]
+    d = (E2[far] - P2[far])
+    assert abs(d).max() == 0
