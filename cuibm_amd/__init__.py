@@ -39,7 +39,10 @@ INFO_KEYS = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "
              # 5-point slots a ghost row leaves unwritten, hybrid tail slots never written,
              # tail slots written twice, interpolation corners outside the 3x3 stencil
              "emb1_unwritten", "emb1_tail_unwritten", "emb1_tail_multi", "emb1_dropped",
-             "emb2_unwritten", "emb2_tail_unwritten", "emb2_tail_multi", "emb2_dropped"]
+             "emb2_unwritten", "emb2_tail_unwritten", "emb2_tail_multi", "emb2_dropped",
+             # passes of the last step (strongly coupled VIV repeats the step), transverse
+             # velocity of the body centre (VIV)
+             "sc_passes", "centerV"]
 
 
 class CuibmError(RuntimeError):
@@ -198,8 +201,8 @@ class Solver:
         _check(lib().cuibm_set_field(self._h, name.encode(), a.ctypes.data_as(C.c_void_p), a.size))
 
     def info(self):
-        out = np.zeros(40)
-        _check(lib().cuibm_info(self._h, _dp(out), 40))
+        out = np.zeros(42)
+        _check(lib().cuibm_info(self._h, _dp(out), 42))
         return dict(zip(INFO_KEYS, out[: len(INFO_KEYS)]))
 
     def bytes(self, which=0):

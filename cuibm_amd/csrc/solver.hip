@@ -332,7 +332,7 @@ long NavierStokesSolver::setField(const std::string &n, const void *in, long cnt
 }
 
 void NavierStokesSolver::info(double *out, int n) const {
-    std::vector<double> v(40, 0.0);
+    std::vector<double> v(42, 0.0);
     v[0] = timeStep;
     v[1] = iterationCount1;
     v[2] = iterationCount2;
@@ -350,7 +350,7 @@ void NavierStokesSolver::info(double *out, int n) const {
     v[29] = bytesTotal2;
     v[30] = solvesTotal2;
     v[31] = restartsTotal;
-    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 40));
+    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 42));
 }
 
 int NavierStokesSolver::phase(const std::string &n) {
@@ -506,7 +506,6 @@ void luo_base::initialise() {
     initialiseNoBody();
     hipStream_t s = stream_;
     if (db->bodies.empty()) throw Error(std::string(solverTypeName(db->solverType)) + " needs a body (bodies.yaml)", ERR_INPUT);
-    if (db->VIV != 0) throw Error("VIV body motion is not supported", ERR_UNSUPPORTED);
     bX.clear();
     bY.clear();
     for (auto &b : db->bodies) {
@@ -522,7 +521,8 @@ void luo_base::initialise() {
     // luo_base.cu:44-85: prescribed in-line oscillation x = xC sin(2 pi f t + xPhase),
     // u = uC cos(2 pi f t + uPhase) about the centroid; the points keep their offsets
     const BodyDesc &b0 = db->bodies[0];
-    moving = b0.xCoefficient != 0 || b0.uCoefficient != 0;
+    moving = b0.xCoefficient != 0 || b0.uCoefficient != 0 || db->VIV != 0;
+    if (db->VIV < 0 || db->VIV > 2) throw Error("VIV must be 0, 1 or 2", ERR_INPUT);
     if (moving && db->solverType == FADLUN)
         throw Error("a moving body with the modified Fadlun method is not supported (its LHS is not rebuilt)",
                     ERR_UNSUPPORTED);
@@ -533,8 +533,10 @@ void luo_base::initialise() {
     midX = xnew;
     midY = my;
     bDX.resize(n);
+    bDY.resize(n);
     for (int i = 0; i < n; i++) {
         bDX[i] = bX[i] - mx;
+        bDY[i] = bY[i] - my;
         bX[i] = midX + bDX[i];
     }
     bUB.assign(n, unew);
@@ -658,13 +660,32 @@ void luo_base::buildStencils() {
     CUIBM_HIP(hipStreamSynchronize(s));
 }
 
-// luo_base.cu:208-233 (VIV strong coupling is not supported)
+// luo_base.cu:208-233. With strongly coupled VIV the step's phases repeat until the body
+// position settles (at least two passes); every pass after the first restarts from
+// u = uold, the velocity the previous pass held before its projection. The reference
+// loops without a limit; here 200 passes end the step with ERR_NOT_CONVERGED.
 void luo_base::stepTime() {
     _pre_step();
-    _intermediate_velocity();
-    _pressure();
-    _project_velocity();
-    _update_body();
+    auto pass = [&] {
+        _intermediate_velocity();
+        _pressure();
+        _project_velocity();
+        _update_body();
+    };
+    if (db->VIV == 2) {
+        scTol = 0.0001;
+        scPasses = 0;
+        while (std::fabs(scTol) > 0.001 || scPasses < 2) {
+            if (scPasses >= 200) throw Error("strongly coupled VIV passes did not settle", ERR_NOT_CONVERGED);
+            if (scPasses > 0)
+                CUIBM_HIP(hipMemcpyAsync(u.p(), uold.p(), sizeof(double) * numUV, hipMemcpyDeviceToDevice, stream_));
+            pass();
+            scPasses++;
+        }
+    } else {
+        scPasses = 1;
+        pass();
+    }
     _post_step();
 }
 
@@ -676,6 +697,8 @@ void luo_base::_pre_step() {
     vB0 = bVB.empty() ? 0 : bVB[0];
     uBk = uB0;
     vBk = vB0;
+    midY0 = midY;
+    centerV0 = centerV;
 }
 
 // luo_base/projectVelocity.cu:11-53
@@ -695,10 +718,14 @@ void luo_base::_update_body() {
 
 void luo_base::_post_step() { timeStep++; }
 
-// luo_base.cu:154-174: force, then the prescribed motion (a static body here)
+// luo_base.cu:154-174: force, then the body's motion
 void luo_base::moveBody() {
     calculateForce();
     if (!moving) return;
+    if (db->VIV != 0) {
+        vivMovement(db->VIV == 2);
+        return;
+    }
     // luo_base/move.cu:11-36 (set_movement) at t = dt * timeStep, before the step counter advances
     const BodyDesc &b0 = db->bodies[0];
     const double t = dt * timeStep;
@@ -711,6 +738,35 @@ void luo_base::moveBody() {
     }
     dbx.upload(bX, stream_);
     dbu.upload(bUB, stream_);
+}
+
+// luo_base/move.cu:38-93 (viv_movement_LC / viv_movement_SC): transverse spring-mass
+// response to the lift (C_L = 2 Fy, reduced mass 2, integer reduced velocity); the strongly
+// coupled form relaxes the position by 0.9 and reports its relative change in scTol
+void luo_base::vivMovement(bool strong) {
+    readForces();
+    const double Cy = forceY * 2.0, Mred = 2.0;
+    const int Ured = db->Ured;
+    const double vold = centerV0, yold = midY0;
+    const double a = dt * M_PI * M_PI * 4 / (Ured * Ured), b = dt * dt * 2 * M_PI * M_PI / (Ured * Ured);
+    const double vnew = (vold - a * (yold + dt / 2 * vold) + dt * Cy / 2 / Mred) / (1 + b);
+    double ynew;
+    if (strong) {
+        const double relax = 0.9;
+        scTol = midY;
+        ynew = relax * (yold + dt / 2 * (vnew + vold)) + (1 - relax) * midY;
+    } else {
+        ynew = yold + dt / 2 * (vnew + vold);
+    }
+    centerV = vnew;
+    midY = ynew;
+    for (size_t i = 0; i < bY.size(); i++) {
+        bY[i] = midY + bDY[i];
+        bVB[i] = vnew;
+    }
+    dby.upload(bY, stream_);
+    dbv.upload(bVB, stream_);
+    if (strong) scTol = (scTol - ynew) / scTol;
 }
 
 // luo_base.cu:137-147: bounding box and tags for the moved body. For a static body the
@@ -812,8 +868,8 @@ long luo_base::getField(const std::string &n, void *out, long cap) {
 
 void luo_base::info(double *out, int n) const {
     const_cast<luo_base *>(this)->readForces();
-    double v[40];
-    NavierStokesSolver::info(v, 40);
+    double v[42];
+    NavierStokesSolver::info(v, 42);
     v[5] = forceX;
     v[6] = forceY;
     v[7] = fxx;
@@ -830,7 +886,9 @@ void luo_base::info(double *out, int n) const {
     v[23] = gnP.n;
     v[24] = hnP.n;
     v[25] = nsolid;
-    std::memcpy(out, v, sizeof(double) * std::min(n, 40));
+    v[40] = scPasses;
+    v[41] = centerV;
+    std::memcpy(out, v, sizeof(double) * std::min(n, 42));
 }
 
 int luo_base::phase(const std::string &n) {
@@ -1062,10 +1120,10 @@ int luoIter::phase(const std::string &n) {
 }
 
 void luoIter::info(double *out, int n) const {
-    double v[40];
-    luo_base::info(v, 40);
+    double v[42];
+    luo_base::info(v, 42);
     for (int k = 0; k < 8; k++) v[32 + k] = (double)embStats[k];
-    std::memcpy(out, v, sizeof(double) * std::min(n, 40));
+    std::memcpy(out, v, sizeof(double) * std::min(n, 42));
 }
 
 // ============================================================================ modified Fadlun

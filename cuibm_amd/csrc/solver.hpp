@@ -123,6 +123,7 @@ class luo_base : public NavierStokesSolver {
     virtual void _update_body();
     virtual void _post_step();
     void moveBody();
+    void vivMovement(bool strong);
     virtual void updateSolver();
     void tagPoints();
     void calculateForce();
@@ -136,9 +137,12 @@ class luo_base : public NavierStokesSolver {
   protected:
     // body state (src/bodies.h)
     std::vector<double> bX, bY, bUB, bVB;  // host copies of the body points / velocities
-    std::vector<double> bDX;               // point offsets from the body centre (x)
-    bool moving = false;                   // prescribed oscillation (nonzero x or u coefficient)
+    std::vector<double> bDX, bDY;          // point offsets from the body centre
+    bool moving = false;                   // prescribed oscillation or VIV
     double midX = 0, midY = 0, uB0 = 0, vB0 = 0, uBk = 0, vBk = 0;
+    double midY0 = 0, centerV = 0, centerV0 = 0;  // transverse motion (VIV)
+    double scTol = 0;                      // strongly coupled VIV: relative position change
+    int scPasses = 0;                      // passes of the last step
     int startI = 0, startJ = 0, numCellsX = 0, numCellsY = 0;
     double forceX = 0, forceY = 0, fxx = 0, fxy = 0, fxu = 0;
     DArray<double> dbx, dby, dbu, dbv;

@@ -125,8 +125,9 @@ long cuibm_set_field(cuibm_solver *s, const char *name, const void *host, long n
  * breakdown restarts; OSC_CYLINDER only, summed over all assemblies of LHS1 then
  * LHS2: [32|36] 5-point slots a ghost row leaves unwritten, [33|37] hybrid tail slots
  * never written, [34|38] tail slots written twice, [35|39] interpolation corners outside
- * the 3x3 stencil (the reference's result depends on stale slots when any is nonzero).
- * n may be up to 40. */
+ * the 3x3 stencil (the reference's result depends on stale slots when any is nonzero);
+ * immersed-body solvers: [40] passes of the last step (above 1 only for strongly coupled
+ * VIV), [41] transverse velocity of the body centre. n may be up to 42. */
 int cuibm_info(cuibm_solver *s, double *out, int n);
 
 /* algorithmic HBM bytes of the last step: which = 0 whole step, 1 velocity solve,

@@ -98,14 +98,18 @@ struct Params {
     double scaleCV = 2.0;
     // prescribed in-line oscillation of the body (src/io/parseBodiesFile.cu:60-104)
     double xCoeff = 0, uCoeff = 0, xPhase = 0, uPhase = 0, xfreq = 0;
+    // flow-induced transverse motion (src/io/parseSimulationFile.cu:80-142): 0 none,
+    // 1 loosely coupled, 2 strongly coupled; reduced velocity (an integer there)
+    int viv = 0, ured = 3;
 };
 
 struct Body {
     int n = 0;
     std::vector<double> x, y, uB, vB, uB0, vB0, uBk, vBk;
-    std::vector<double> dx;  // point offsets from the body centre (x)
+    std::vector<double> dx, dy;  // point offsets from the body centre
     bool moving = false;
-    double midX = 0, midY = 0;
+    double midX = 0, midY = 0, midY0 = 0;
+    double centerV = 0, centerV0 = 0;  // transverse velocity of the body centre (VIV)
     int startI = 0, startJ = 0, numCellsX = 0, numCellsY = 0;
     double forceX = 0, forceY = 0;
 };
@@ -176,9 +180,18 @@ struct Solver {
     void step_ns();
     void step_luo();
     void move_body();
+    // luo_base::stepTime (luo_base.cu:208-285): pre-step, one pass (or strongly coupled
+    // passes until the body position settles), step counter
+    void luo_pre_step();
+    void luo_pass();
+    void update_body();
+    void viv_move(bool strong);
+    int sc_passes = 0;      // passes of the last step
+    double sc_tol = 0;
     void step_fadlun();
     // Luo embedded
     void step_embedded();
+    void embedded_pass();
     void alpha_uv();
     void alpha_pressure();
     void buildLHS1_embedded();

@@ -370,12 +370,8 @@ void Solver::update_rhs2_embedded() {
     }
 }
 
-// luo_base.cu:208-264 with luo_iter.cu:37-47
-void Solver::step_embedded() {
-    p_old = p;
-    uold = u;
-    Nold = N;
-    B.uB0 = B.uB; B.vB0 = B.vB; B.uBk = B.uB; B.vBk = B.vB;
+// one pass of luo_base::stepTime for the embedded method (luo_iter.cu:37-47)
+void Solver::embedded_pass() {
     // intermediate_velocity_setup (luo_iter/intermediateVelocity.cu:17-77)
     u = uold;
     robin();
@@ -404,15 +400,10 @@ void Solver::step_embedded() {
     project_nobody();
     gn_velocity(u, true);
     set_inside(u);
-    // luo_base.cu:154-174 and 137-147
-    force();
-    if (B.moving) {
-        move_body();
-        boundingBox();
-        tagPoints();
-    }
-    timeStep++;
+    update_body();
 }
+
+void Solver::step_embedded() { step_luo(); }
 
 }  // namespace orc
 

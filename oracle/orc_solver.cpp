@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <stdexcept>
 #include "orc.hpp"
 
 namespace orc {
@@ -92,7 +93,7 @@ void Solver::init(const Params &Pin, const Grid &Gin, const std::vector<double> 
         my /= B.n;
         // luo_base.cu:44-85: position and velocity of the prescribed oscillation at t = dt * timeStep,
         // the previous velocity at t - dt; the points keep their offsets from the centroid
-        B.moving = P.xCoeff != 0 || P.uCoeff != 0;
+        B.moving = P.xCoeff != 0 || P.uCoeff != 0 || P.viv != 0;
         const double t = P.dt * timeStep;
         const double xnew = P.xCoeff * std::sin(2 * M_PI * P.xfreq * t + P.xPhase);
         const double unew = P.uCoeff * std::cos(2 * M_PI * P.xfreq * t + P.uPhase);
@@ -100,8 +101,10 @@ void Solver::init(const Params &Pin, const Grid &Gin, const std::vector<double> 
         B.midX = xnew;
         B.midY = my;
         B.dx.resize(B.n);
+        B.dy.resize(B.n);
         for (int i = 0; i < B.n; i++) {
             B.dx[i] = B.x[i] - mx;
+            B.dy[i] = B.y[i] - my;
             B.x[i] = B.midX + B.dx[i];
         }
         B.uB.assign(B.n, unew);
@@ -144,12 +147,18 @@ void Solver::step_ns() {
     timeStep++;
 }
 
-// luo_base.cu:208-285 with luoIBM.cu:91-102
-void Solver::step_luo() {
+// luo_base.cu:236-248
+void Solver::luo_pre_step() {
     p_old = p;
     uold = u;
     Nold = N;
     B.uB0 = B.uB; B.vB0 = B.vB; B.uBk = B.uB; B.vBk = B.vB;
+    B.midY0 = B.midY;
+    B.centerV0 = B.centerV;
+}
+
+// one pass of luo_base::stepTime for the external method (luoIBM.cu:91-102)
+void Solver::luo_pass() {
     // luoIBM/intermediateVelocity.cu:18-57
     u = uold;
     robin();
@@ -174,13 +183,35 @@ void Solver::step_luo() {
     project_nobody();
     gn_velocity(u, true);
     set_inside(u);
-    // luo_base.cu:154-174 and 137-147: force, prescribed motion, bounding box and re-tag
-    // (for a static body the re-tag reproduces the tags and is skipped)
+    update_body();
+}
+
+// luo_base.cu:154-174 and 137-147: force, the body's motion, bounding box and re-tag
+// (for a static body the re-tag reproduces the tags and is skipped)
+void Solver::update_body() {
     force();
-    if (B.moving) {
-        move_body();
-        boundingBox();
-        tagPoints();
+    if (!B.moving) return;
+    if (P.viv == 0) move_body();
+    else viv_move(P.viv == 2);
+    boundingBox();
+    tagPoints();
+}
+
+// luo_base.cu:208-233 (the strongly coupled loop stops after 200 passes here; the
+// reference loops until the position settles)
+void Solver::step_luo() {
+    luo_pre_step();
+    if (P.viv == 2) {
+        sc_tol = 0.0001;
+        sc_passes = 0;
+        while (std::fabs(sc_tol) > 0.001 || sc_passes < 2) {
+            if (sc_passes >= 200) throw std::runtime_error("strongly coupled VIV passes did not settle");
+            P.solver_type == OSC ? embedded_pass() : luo_pass();
+            sc_passes++;
+        }
+    } else {
+        sc_passes = 1;
+        P.solver_type == OSC ? embedded_pass() : luo_pass();
     }
     timeStep++;
 }
@@ -195,6 +226,32 @@ void Solver::move_body() {
         B.uB[i] = unew;
         B.x[i] = B.midX + B.dx[i];
     }
+}
+
+// luo_base/move.cu:38-93 (viv_movement_LC / viv_movement_SC): transverse spring-mass
+// response to the lift, reduced mass 2; the strongly coupled form relaxes the position
+// and reports its relative change in sc_tol
+void Solver::viv_move(bool strong) {
+    const double Cy = B.forceY * 2.0, Mred = 2.0;
+    const int Ured = P.ured;
+    const double vold = B.centerV0, yold = B.midY0, dt = P.dt;
+    const double a = dt * M_PI * M_PI * 4 / (Ured * Ured), b = dt * dt * 2 * M_PI * M_PI / (Ured * Ured);
+    const double vnew = (vold - a * (yold + dt / 2 * vold) + dt * Cy / 2 / Mred) / (1 + b);
+    double ynew;
+    if (strong) {
+        const double relax = 0.9;
+        sc_tol = B.midY;
+        ynew = relax * (yold + dt / 2 * (vnew + vold)) + (1 - relax) * B.midY;
+    } else {
+        ynew = yold + dt / 2 * (vnew + vold);
+    }
+    B.centerV = vnew;
+    B.midY = ynew;
+    for (int i = 0; i < B.n; i++) {
+        B.vB[i] = vnew;
+        B.y[i] = B.midY + B.dy[i];
+    }
+    if (strong) sc_tol = (sc_tol - ynew) / sc_tol;
 }
 
 // luo_base.cu:208-285 with fadlunModified.cu:97-124
@@ -219,7 +276,7 @@ void Solver::step_fadlun() {
 
 void Solver::step() {
     if (P.solver_type == LUO && has_body) step_luo();
-    else if (P.solver_type == OSC && has_body) step_embedded();
+    else if (P.solver_type == OSC && has_body) step_luo();
     else if (P.solver_type == FADLUN && has_body) step_fadlun();
     else step_ns();
 }
@@ -321,6 +378,7 @@ struct orc_params {
     int nbody;
     const double *bx, *by;
     double xCoeff, uCoeff, xPhase, uPhase, xfreq;
+    int viv, ured;
 };
 
 void *orc_create(const orc_params *q) {
@@ -346,6 +404,8 @@ void *orc_create(const orc_params *q) {
     P.xPhase = q->xPhase;
     P.uPhase = q->uPhase;
     P.xfreq = q->xfreq;
+    P.viv = q->viv;
+    P.ured = q->ured;
     orc::Grid G;
     G.nx = q->nx;
     G.ny = q->ny;
@@ -452,6 +512,8 @@ void orc_info(void *h, double *out) {
     out[14] = S->B.midX;
     out[15] = S->B.midY;
     out[16] = S->PC2.type == orc::P_SMOOTHED_AGGREGATION ? (double)S->PC2.amg.lv.size() : 0;
+    out[17] = S->sc_passes;
+    out[18] = S->B.centerV;
 }
 
 // standalone linear-algebra entry points used by the solver tests

@@ -38,7 +38,7 @@ class OrcParams(C.Structure):
         ("maxit1", C.c_int), ("maxit2", C.c_int),
         ("scaleCV", C.c_double), ("nbody", C.c_int), ("bx", _DP), ("by", _DP),
         ("xCoeff", C.c_double), ("uCoeff", C.c_double), ("xPhase", C.c_double), ("uPhase", C.c_double),
-        ("xfreq", C.c_double),
+        ("xfreq", C.c_double), ("viv", C.c_int), ("ured", C.c_int),
     ]
 
 
@@ -121,10 +121,11 @@ def _load(folder, name):
 def read_case(folder, overrides=None):
     """Parse the four case files into a plain dict (reference keys and defaults)."""
     sim = {"dt": 0.02, "scaleCV": 2.0, "nt": 100, "nsave": 100, "SolverType": "NAVIER_STOKES_SOLVER",
+           "VIV": 0, "Ured": 3,
            "velocity": {"preconditioner": "DIAGONAL", "tolerance": 1e-5, "maxIterations": 10000},
            "Poisson": {"preconditioner": "DIAGONAL", "tolerance": 1e-5, "maxIterations": 20000}}
     for node in _load(folder, "simParams.yaml"):
-        for k in ("dt", "scaleCV", "nt", "nsave", "SolverType"):
+        for k in ("dt", "scaleCV", "nt", "nsave", "SolverType", "VIV", "Ured"):
             if k in node:
                 sim[k] = node[k]
         prec, tol, mit = "DIAGONAL", 1e-5, 10000
@@ -216,6 +217,7 @@ class Oracle:
         m = case.get("motion", {})
         p.xCoeff, p.uCoeff = m.get("xCoeff", 0.0), m.get("uCoeff", 0.0)
         p.xPhase, p.uPhase, p.xfreq = m.get("xPhase", 0.0), m.get("uPhase", 0.0), m.get("xfreq", 0.0)
+        p.viv, p.ured = int(s.get("VIV", 0)), int(s.get("Ured", 3))
         if case["bodies"]:
             bx = np.concatenate([b[0] for b in case["bodies"]])
             by = np.concatenate([b[1] for b in case["bodies"]])
@@ -261,7 +263,7 @@ class Oracle:
         out = np.zeros(32)
         lib().orc_info(self.h, dptr(out))
         keys = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "fxx", "fxy", "fxu",
-                "startI", "startJ", "numCellsX", "numCellsY", "midX", "midY", "amg_levels"]
+                "startI", "startJ", "numCellsX", "numCellsY", "midX", "midY", "amg_levels", "sc_passes", "centerV"]
         return dict(zip(keys, out[:len(keys)]))
 
     def embedded_stats(self):

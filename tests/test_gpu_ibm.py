@@ -76,3 +76,28 @@ def test_oscillating_cylinder_matches_oracle():
             assert abs(gi[f] - oi[f]) <= 1e-8 * max(1.0, abs(oi[f])), (k, f, gi[f], oi[f])
     # the body has moved by more than a cell, so the re-tag path was exercised
     assert not np.array_equal(o.get("ghostTagsUV"), tags0)
+
+
+@pytest.mark.parametrize("name,steps", [("cylinderRe40_small_viv_lc", 8), ("cylinderRe40_small_viv_sc", 6),
+                                        ("cylinderRe40_small_embedded_viv_sc", 5)])
+def test_viv_matches_oracle(name, steps):
+    """Flow-induced transverse motion (luo_base/move.cu:38-93): the lift drives a spring-mass
+    body; VIV 2 repeats the step (restarting from the pre-projection velocity) until the
+    position settles. Checked after every step: passes, body centre and velocity, tags,
+    u and p, forces."""
+    g, o = pair(name, {"velocityTol": 1e-12, "poissonTol": 1e-13})
+    for k in range(steps):
+        g.step_time()
+        o.step()
+        gi, oi = g.info(), o.info()
+        assert gi["sc_passes"] == oi["sc_passes"], (k, gi["sc_passes"], oi["sc_passes"])
+        assert abs(gi["midY"] - oi["midY"]) <= 1e-10 * max(1e-3, abs(oi["midY"])), (k, gi["midY"], oi["midY"])
+        assert abs(gi["centerV"] - oi["centerV"]) <= 1e-9 * max(1e-3, abs(oi["centerV"])), k
+        for t in TAGS:
+            assert np.array_equal(g.get(t), o.get(t)), (k, t)
+        assert rel_l2(g.get("u"), o.get("u")) < 1e-10, k
+        pg, po_ = g.get("p"), o.get("p")
+        assert rel_l2(pg - pg.mean(), po_ - po_.mean()) < 1e-10, k
+        for f in ("forceX", "forceY"):
+            assert abs(gi[f] - oi[f]) <= 1e-8 * max(1.0, abs(oi[f])), (k, f, gi[f], oi[f])
+    assert g.info()["midY"] != 0
