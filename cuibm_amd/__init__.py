@@ -42,7 +42,9 @@ INFO_KEYS = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "
              "emb2_unwritten", "emb2_tail_unwritten", "emb2_tail_multi", "emb2_dropped",
              # passes of the last step (strongly coupled VIV repeats the step), transverse
              # velocity of the body centre (VIV)
-             "sc_passes", "centerV"]
+             "sc_passes", "centerV",
+             # largest CFL number so far (NavierStokes/CFL.cu:10-35), its cell and time step
+             "cfl_max", "cfl_I", "cfl_J", "cfl_ts"]
 
 
 class CuibmError(RuntimeError):
@@ -84,6 +86,8 @@ def lib():
     L.cuibm_set_field.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_long]
     L.cuibm_set_field.restype = C.c_long
     L.cuibm_info.argtypes = [C.c_void_p, _DP, C.c_int]
+    L.cuibm_timers.restype = C.c_long
+    L.cuibm_timers.argtypes = [C.c_void_p, C.c_char_p, C.c_long]
     L.cuibm_bytes.argtypes = [C.c_void_p, C.c_int]
     L.cuibm_device_synchronize.argtypes = []
     L.cuibm_device_synchronize.restype = C.c_int
@@ -201,9 +205,21 @@ class Solver:
         _check(lib().cuibm_set_field(self._h, name.encode(), a.ctypes.data_as(C.c_void_p), a.size))
 
     def info(self):
-        out = np.zeros(42)
-        _check(lib().cuibm_info(self._h, _dp(out), 42))
+        out = np.zeros(46)
+        _check(lib().cuibm_info(self._h, _dp(out), 46))
         return dict(zip(INFO_KEYS, out[: len(INFO_KEYS)]))
+
+    def timers(self):
+        """Accumulated wall-clock seconds per phase, with the reference's event names."""
+        n = lib().cuibm_timers(self._h, None, 0)
+        _check(n if n < 0 else 0)
+        buf = C.create_string_buffer(int(n) + 1)
+        lib().cuibm_timers(self._h, buf, n + 1)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, sec = line.rsplit("\t", 1)
+            out[name] = float(sec)
+        return out
 
     def bytes(self, which=0):
         return lib().cuibm_bytes(self._h, which)

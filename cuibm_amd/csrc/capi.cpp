@@ -1,6 +1,7 @@
 // extern "C" boundary (include/cuibm_amd.h). Exceptions never cross it: every entry
 // point catches cuibm::Error / std::exception and returns a negative code.
 #include <cstring>
+#include <sstream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -236,6 +237,21 @@ int cuibm_info(cuibm_solver *s, double *out, int n) {
         need(s);
         s->s->info(out, n);
         return 0;
+    });
+}
+
+long cuibm_timers(cuibm_solver *s, char *buf, long n) {
+    return guardl([&]() -> long {
+        need(s);
+        std::ostringstream os;
+        for (const auto &e : s->s->timers().timers()) os << e.first << '\t' << e.second << '\n';
+        const std::string t = os.str();
+        if (buf && n > 0) {
+            const long c = std::min<long>(n - 1, (long)t.size());
+            std::memcpy(buf, t.data(), (size_t)c);
+            buf[c] = '\0';
+        }
+        return (long)t.size();
     });
 }
 

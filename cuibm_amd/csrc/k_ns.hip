@@ -245,6 +245,74 @@ void launch_project(double *u, const double *uhat, double *uold, const double *p
     CUIBM_LAUNCH_CHECK();
 }
 
+// ---- CFL monitor (NavierStokes/kernels/CFL.cu:10-23 + the thrust::max_element of
+// NavierStokes/CFL.cu:19-21): per-cell dt (|u|/dx + |v|/dy), its maximum and the first
+// index attaining it; the reference never writes the last column and row of its array,
+// which stay zero
+__device__ __forceinline__ void argmax_merge(double &v, int &i, double v2, int i2) {
+    if (v2 > v || (v2 == v && i2 < i)) {
+        v = v2;
+        i = i2;
+    }
+}
+
+__device__ void block_argmax(double &best, int &bi) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_down(best, o, 64);
+        const int i2 = __shfl_down(bi, o, 64);
+        argmax_merge(best, bi, v2, i2);
+    }
+    __shared__ double sv[BS / 64];
+    __shared__ int si[BS / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sv[wave] = best;
+        si[wave] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < BS / 64; w++) argmax_merge(best, bi, sv[w], si[w]);
+}
+
+__global__ void __launch_bounds__(BS) k_cfl_part(const double *__restrict__ u, GridDev g, double dt,
+                                                 double *__restrict__ pv, int *__restrict__ pi) {
+    const int nx = g.nx, ny = g.ny, n = nx * ny, numU = (nx - 1) * ny;
+    double best = -1.0;
+    int bi = n;
+    for (int ip = blockIdx.x * blockDim.x + threadIdx.x; ip < n; ip += gridDim.x * blockDim.x) {
+        const int I = ip % nx, J = ip / nx;
+        double c = 0;
+        if (I != nx - 1 && J != ny - 1)
+            c = dt * (fabs(u[(nx - 1) * J + I]) / g.dx[I] + fabs(u[numU + nx * J + I]) / g.dy[J]);
+        argmax_merge(best, bi, c, ip);
+    }
+    block_argmax(best, bi);
+    if (threadIdx.x == 0) {
+        pv[blockIdx.x] = best;
+        pi[blockIdx.x] = bi;
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_cfl_final(const double *__restrict__ pv, const int *__restrict__ pi, int nb,
+                                                  double *__restrict__ out) {
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) argmax_merge(best, bi, pv[k], pi[k]);
+    block_argmax(best, bi);
+    if (threadIdx.x == 0) {
+        out[0] = best;
+        out[1] = bi;
+    }
+}
+
+void launch_cfl(const double *u, const GridDev &g, double dt, double *part, double *out, hipStream_t s) {
+    int *pi = reinterpret_cast<int *>(part + CFL_BLOCKS);
+    hipLaunchKernelGGL(k_cfl_part, dim3(CFL_BLOCKS), dim3(BS), 0, s, u, g, dt, part, pi);
+    CUIBM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(BS), 0, s, part, pi, CFL_BLOCKS, out);
+    CUIBM_LAUNCH_CHECK();
+}
+
 void launch_robin(const double *u, double *xp, const GridDev &g, double dt, hipStream_t s) {
     hipLaunchKernelGGL(k_robin, dim3(nblk(2 * g.ny - 1)), dim3(BS), 0, s, u, xp, g, dt);
     CUIBM_LAUNCH_CHECK();

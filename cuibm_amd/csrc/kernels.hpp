@@ -46,6 +46,10 @@ void launch_rhs2(const double *uhat, double *rhs2, const GridDev &g, const BcDev
 void launch_project(double *u, const double *uhat, double *uold, const double *p, const GridDev &g, double dt,
                     const int *ghostP, hipStream_t s);
 void launch_robin(const double *u, double *xp, const GridDev &g, double dt, hipStream_t s);
+// maximum CFL number over the cells and the first cell index attaining it: out[0], out[1];
+// part holds CFL_BLOCKS doubles then CFL_BLOCKS ints
+constexpr int CFL_BLOCKS = 256;
+void launch_cfl(const double *u, const GridDev &g, double dt, double *part, double *out, hipStream_t s);
 
 // ---- k_lhs.hip
 void lhs1_rowptr(int nx, int ny, std::vector<int> &rp);

@@ -35,6 +35,7 @@ BcDev NavierStokesSolver::bcs() const {
 std::string NavierStokesSolver::outFolder() const { return opt.outputFolder.empty() ? db->caseFolder : opt.outputFolder; }
 
 void NavierStokesSolver::initialise() {
+    ScopedTimer t(logger, "initialise");
     initialiseNoBody();
     initialiseLHS();
 }
@@ -189,26 +190,53 @@ void NavierStokesSolver::stepTime() {
     generateRHS2();
     solvePoisson();
     velocityProjection();
+    CFL();
     timeStep++;
+}
+
+// NavierStokes/CFL.cu:10-35: the maximum over the cells; a new maximum records its cell
+// and step (the reference also warns on stdout when it jumps by 20% after step 100)
+void NavierStokesSolver::CFL() {
+    ScopedTimer t(logger, "CFL");
+    cflPart.alloc(2 * CFL_BLOCKS);
+    cflOut.alloc(2);
+    launch_cfl(u.p(), grid(), dt, cflPart.p(), cflOut.p(), stream_);
+    double m[2];
+    cflOut.download(m, 2, stream_);
+    if (m[0] > cfl_max) {
+        cfl_max = m[0];
+        const int position = (int)m[1];
+        cfl_I = position % nx;
+        cfl_J = position / nx;
+        cfl_ts = timeStep;
+    }
 }
 
 // NavierStokes/intermediateVelocity.cu:14-38 (Nold = N is a buffer swap)
 void NavierStokesSolver::generateRHS1() {
+    ScopedTimer t(logger, "Intermediate Velocity Setup");
     std::swap(N, Nold);
     launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
                 opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
 }
 
-void NavierStokesSolver::generateRHS2() { launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_); }
+void NavierStokesSolver::generateRHS2() {
+    ScopedTimer t(logger, "Poisson Setup");
+    launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_);
+}
 
 void NavierStokesSolver::projectPlain() {
     launch_project(u.p(), uhat.p(), uold.p(), pressure.p(), grid(), dt, nullptr, stream_);
 }
 
-void NavierStokesSolver::velocityProjection() { projectPlain(); }
+void NavierStokesSolver::velocityProjection() {
+    ScopedTimer t(logger, "Velocity Projection");
+    projectPlain();
+}
 
 // NavierStokesSolver.cu:221-271
 void NavierStokesSolver::solveIntermediateVelocity() {
+    ScopedTimer t(logger, "Intermediate Velocity Solve");
     stat1 = ks1->solve(*op1, uhat.p(), rhs1.p(), *PC1, db->velocitySolve.tolerance, db->velocitySolve.maxIterations,
                        stream_);
     iterationCount1 = stat1.iterations;
@@ -224,6 +252,7 @@ void NavierStokesSolver::solveIntermediateVelocity() {
 }
 
 void NavierStokesSolver::solvePoisson() {
+    ScopedTimer t(logger, "Poisson Solve");
     stat2 = ks2->solve(*op2, pressure.p(), rhs2.p(), *PC2, db->PoissonSolve.tolerance, db->PoissonSolve.maxIterations,
                        stream_);
     iterationCount2 = stat2.iterations;
@@ -246,11 +275,19 @@ void NavierStokesSolver::writeCommon() {
     iterationsFile << timeStep << '\t' << iterationCount1 << '\t' << iterationCount2 << std::endl;
 }
 
-void NavierStokesSolver::writeData() { writeCommon(); }
+void NavierStokesSolver::writeData() {
+    ScopedTimer t(logger, "output");
+    writeCommon();
+}
 
+// NavierStokesSolver.cu:368-372 with Logger::writeTime: the phase timers go to <folder>/time
 void NavierStokesSolver::shutDown() {
     if (iterationsFile.is_open()) iterationsFile.close();
     CUIBM_HIP(hipStreamSynchronize(stream_));
+    if (opt.output) {
+        std::ofstream f(outFolder() + "/time");
+        logger.write(f);
+    }
 }
 
 // ---- field access
@@ -332,7 +369,7 @@ long NavierStokesSolver::setField(const std::string &n, const void *in, long cnt
 }
 
 void NavierStokesSolver::info(double *out, int n) const {
-    std::vector<double> v(42, 0.0);
+    std::vector<double> v(46, 0.0);
     v[0] = timeStep;
     v[1] = iterationCount1;
     v[2] = iterationCount2;
@@ -350,7 +387,11 @@ void NavierStokesSolver::info(double *out, int n) const {
     v[29] = bytesTotal2;
     v[30] = solvesTotal2;
     v[31] = restartsTotal;
-    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 42));
+    v[42] = cfl_max;
+    v[43] = cfl_I;
+    v[44] = cfl_J;
+    v[45] = cfl_ts;
+    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 46));
 }
 
 int NavierStokesSolver::phase(const std::string &n) {
@@ -503,6 +544,7 @@ void luo_base::boundingBox() {
 
 // luo_base.cu:28-107 with src/bodies.cu:20-185
 void luo_base::initialise() {
+    ScopedTimer timer(logger, "initialise");
     initialiseNoBody();
     hipStream_t s = stream_;
     if (db->bodies.empty()) throw Error(std::string(solverTypeName(db->solverType)) + " needs a body (bodies.yaml)", ERR_INPUT);
@@ -703,6 +745,7 @@ void luo_base::_pre_step() {
 
 // luo_base/projectVelocity.cu:11-53
 void luo_base::_project_velocity() {
+    ScopedTimer t(logger, "Velocity Projection");
     projectPlain();
     launch_gn_vel_apply(u.p(), gnV, numU, bUB[0], bVB[0], stream_);
     launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_);
@@ -713,6 +756,7 @@ void luo_base::_update_body() {
     if (timeStep >= db->startStep) {
         moveBody();
         updateSolver();
+        CFL();
     }
 }
 
@@ -722,6 +766,7 @@ void luo_base::_post_step() { timeStep++; }
 void luo_base::moveBody() {
     calculateForce();
     if (!moving) return;
+    ScopedTimer timer(logger, "Move Body");
     if (db->VIV != 0) {
         vivMovement(db->VIV == 2);
         return;
@@ -772,12 +817,14 @@ void luo_base::vivMovement(bool strong) {
 // luo_base.cu:137-147: bounding box and tags for the moved body. For a static body the
 // reference's per-step re-tag reproduces the same tags, so it is only run after a move.
 void luo_base::updateSolver() {
+    ScopedTimer t(logger, "Update Solver");
     if (!moving) return;
     boundingBox();
     tagPoints();
 }
 
 void luo_base::calculateForce() {
+    ScopedTimer t(logger, "Calculate Force");
     launch_force(u.p(), uold.p(), pressure.p(), ghostTagsUV.p(), grid(), nu, dt, startI, startJ, numCellsX, numCellsY,
                  forcePart.p(), forceOut.p(), stream_);
 }
@@ -796,6 +843,7 @@ void luo_base::updateRobinBoundary() { launch_robin(u.p(), bc[XPLUS].p(), grid()
 
 // luo_base.cu:119-128
 void luo_base::writeData() {
+    ScopedTimer t(logger, "output");
     writeCommon();
     if (!opt.output) return;
     readForces();
@@ -868,8 +916,8 @@ long luo_base::getField(const std::string &n, void *out, long cap) {
 
 void luo_base::info(double *out, int n) const {
     const_cast<luo_base *>(this)->readForces();
-    double v[42];
-    NavierStokesSolver::info(v, 42);
+    double v[46];
+    NavierStokesSolver::info(v, 46);
     v[5] = forceX;
     v[6] = forceY;
     v[7] = fxx;
@@ -888,7 +936,7 @@ void luo_base::info(double *out, int n) const {
     v[25] = nsolid;
     v[40] = scPasses;
     v[41] = centerV;
-    std::memcpy(out, v, sizeof(double) * std::min(n, 42));
+    std::memcpy(out, v, sizeof(double) * std::min(n, 46));
 }
 
 int luo_base::phase(const std::string &n) {
@@ -935,6 +983,7 @@ void luoIBM::_pressure() {
 
 // luoIBM/intermediateVelocity.cu:18-57 (u = uold is the identity here: uold was set from u)
 void luoIBM::generateRHS1() {
+    ScopedTimer t(logger, "Intermediate Velocity Setup");
     updateRobinBoundary();
     rhs1GNInterpolation();
     rhs1HNInterpolation();
@@ -951,18 +1000,23 @@ void luoIBM::rhs1HNInterpolation() { launch_hn_vel_apply(u.p(), ustar.p(), hnV, 
 
 void luoIBM::zeroVelocity() { launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_); }
 
-void luoIBM::weightUhat() { launch_weight(uhat.p(), ustar.p(), hnV, stream_); }
+void luoIBM::weightUhat() {
+    ScopedTimer t(logger, "Intermediate Velocity Weight");
+    launch_weight(uhat.p(), ustar.p(), hnV, stream_);
+}
 
 // luoIBM/intermediatePressure.cu:15-29, 79-103
 void luoIBM::preRHS2Interpolation() { launch_gn_vel_apply(uhat.p(), gnV, numU, bUB[0], bVB[0], stream_); }
 
 void luoIBM::generateRHS2() {
+    ScopedTimer t(logger, "Poisson Setup");
     preRHS2Interpolation();
     launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_);
 }
 
 // luoIBM/intermediatePressure.cu:44-77
 void luoIBM::weightPressure() {
+    ScopedTimer t(logger, "Poisson Weight");
     const double dudt = (bUB[0] - uBk) / dt, dvdt = (bVB[0] - vBk) / dt;
     launch_hn_p_apply(pressure.p(), pstar.p(), hnP, dudt, dvdt, stream_);
     launch_weight(pressure.p(), pstar.p(), hnP, stream_);
@@ -1090,19 +1144,25 @@ void luoIter::initialiseLHS() {
 
 // luo_iter/intermediateVelocity.cu:17-77 (u = uold is the identity here) and the solve
 void luoIter::_intermediate_velocity() {
-    updateRobinBoundary();
-    launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_);
-    launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
-                opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
-    launch_emb_rhs1(rhs1.p(), u.p(), asm1(), grid(), bUB[0], bVB[0], stream_);
+    {
+        ScopedTimer t(logger, "Intermediate Velocity Setup");
+        updateRobinBoundary();
+        launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_);
+        launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
+                    opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
+        launch_emb_rhs1(rhs1.p(), u.p(), asm1(), grid(), bUB[0], bVB[0], stream_);
+    }
     solveIntermediateVelocity();
 }
 
 // luo_iter/poisson.cu:15-52 and the solve
 void luoIter::_pressure() {
-    launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_);
-    const double dudt = (bUB[0] - uBk) / dt, dvdt = (bVB[0] - vBk) / dt;
-    launch_emb_rhs2(rhs2.p(), asm2(), grid(), dudt, dvdt, stream_);
+    {
+        ScopedTimer t(logger, "Poisson Setup");
+        launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_);
+        const double dudt = (bUB[0] - uBk) / dt, dvdt = (bVB[0] - vBk) / dt;
+        launch_emb_rhs2(rhs2.p(), asm2(), grid(), dudt, dvdt, stream_);
+    }
     solvePoisson();
 }
 
@@ -1120,10 +1180,10 @@ int luoIter::phase(const std::string &n) {
 }
 
 void luoIter::info(double *out, int n) const {
-    double v[42];
-    luo_base::info(v, 42);
+    double v[46];
+    luo_base::info(v, 46);
     for (int k = 0; k < 8; k++) v[32 + k] = (double)embStats[k];
-    std::memcpy(out, v, sizeof(double) * std::min(n, 42));
+    std::memcpy(out, v, sizeof(double) * std::min(n, 46));
 }
 
 // ============================================================================ modified Fadlun
@@ -1169,20 +1229,26 @@ void fadlunModified::_pressure() {
 
 // FadlunModified/intermediateVelocity.cu:15-38 (updateRHS1forIB fused into the sweep)
 void fadlunModified::generateRHS1() {
+    ScopedTimer t(logger, "Intermediate Velocity Setup");
     updateRobinBoundary();
     launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), opt.keepDiagnostics ? L.p() : nullptr,
                 opt.keepDiagnostics ? bc1.p() : nullptr, grid(), bcs(), nu, dt, fadlunRhs(), stream_);
 }
 
-void fadlunModified::generateRHS2() { launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_); }
+void fadlunModified::generateRHS2() {
+    ScopedTimer t(logger, "Poisson Setup");
+    launch_rhs2(uhat.p(), rhs2.p(), grid(), bcs(), fadlunP2(), stream_);
+}
 
 // fadlunModified.cu:109-124
 void fadlunModified::_project_velocity() {
+    ScopedTimer t(logger, "Velocity Projection");
     launch_project(u.p(), uhat.p(), uold.p(), pressure.p(), grid(), dt, ghostTagsP.p(), stream_);
 }
 
 // fadlunModified.cu:64-95: the header test (timeStep == 0) is never true at write time
 void fadlunModified::writeData() {
+    ScopedTimer t(logger, "output");
     writeCommon();
     calculateForce();
     if (!opt.output) return;

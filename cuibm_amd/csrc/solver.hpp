@@ -12,6 +12,7 @@
 #include "ibm.hpp"
 #include "kernels.hpp"
 #include "krylov.hpp"
+#include "logger.hpp"
 
 namespace cuibm {
 
@@ -68,6 +69,9 @@ class NavierStokesSolver {
     virtual bool stencilLHS2() const { return true; }
     AmgPrecond *amg2() const;  // the Poisson AMG preconditioner (throws if another)
     AmgHierarchyInfo amgInfo() const;
+    // NavierStokesSolver::CFL (NavierStokes/CFL.cu:10-35): track the largest CFL number
+    void CFL();
+    const Logger &timers() const { return logger; }
 
   protected:
     ParameterDB *db;
@@ -80,6 +84,10 @@ class NavierStokesSolver {
     // running totals over all solves since initialise (for throughput accounting)
     double itTotal1 = 0, itTotal2 = 0, bytesTotal1 = 0, bytesTotal2 = 0, solvesTotal2 = 0, restartsTotal = 0;
     SolveStat stat1, stat2;
+    Logger logger;  // the reference's named phase timers
+    double cfl_max = 0;
+    int cfl_I = 0, cfl_J = 0, cfl_ts = 0;
+    DArray<double> cflPart, cflOut;
 
     DArray<double> gx, gdx, gy, gdy, gxu, gyv;
     DArray<double> u, uhat, uold, pressure, N, Nold, L, bc1, rhs1, rhs2, bc[4];

@@ -127,8 +127,17 @@ long cuibm_set_field(cuibm_solver *s, const char *name, const void *host, long n
  * never written, [34|38] tail slots written twice, [35|39] interpolation corners outside
  * the 3x3 stencil (the reference's result depends on stale slots when any is nonzero);
  * immersed-body solvers: [40] passes of the last step (above 1 only for strongly coupled
- * VIV), [41] transverse velocity of the body centre. n may be up to 42. */
+ * VIV), [41] transverse velocity of the body centre; [42] largest CFL number so far,
+ * [43..45] its cell I, J and time step (NavierStokes/CFL.cu:10-35). n may be up to 46. */
 int cuibm_info(cuibm_solver *s, double *out, int n);
+
+/* Logger (src/logger.h:94-232): accumulated host wall-clock seconds per phase, with the
+ * reference's event names ("initialise", "Intermediate Velocity Setup", "Intermediate
+ * Velocity Solve", "Intermediate Velocity Weight", "Poisson Setup", "Poisson Solve",
+ * "Poisson Weight", "Velocity Projection", "Calculate Force", "Move Body", "Update Solver",
+ * "CFL", "output"), written as "name<TAB>seconds" lines into buf (at most n bytes, NUL
+ * terminated); returns the length of the full text. */
+long cuibm_timers(cuibm_solver *s, char *buf, long n);
 
 /* algorithmic HBM bytes of the last step: which = 0 whole step, 1 velocity solve,
  * 2 Poisson solve */

@@ -5,7 +5,8 @@ tightened so that the BiCGSTAB stopping point does not dominate the difference).
 """
 import numpy as np
 import pytest
-from helpers import pair, rel_l2
+import pyoracle
+from helpers import case, pair, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -76,3 +77,33 @@ def test_poisson_stencil_equals_csr_product(name):
     A = cuibm_amd.Csr(sp.csr_matrix((v, ci, rp), shape=(p.size, p.size)))
     y_csr = A.spmv(p)
     assert np.array_equal(y_stencil, y_csr)
+
+
+@pytest.mark.parametrize("name", ["lidDrivenCavityRe100_32", "cylinderRe40_small_luo"])
+def test_cfl_monitor_and_phase_timers(name):
+    """The CFL monitor (NavierStokes/CFL.cu:10-35) keeps the largest dt (|u|/dx + |v|/dy)
+    over the cells (last column and row excluded), with the first cell attaining it and
+    the step; the phase timers carry the reference's event names."""
+    g, o = pair(name, TIGHT)
+    dt = float(pyoracle.read_case(case(name))["sim"]["dt"])
+    dx, dy = o.get("dx"), o.get("dy")
+    nx, ny = len(dx), len(dy)
+    best, cell, step = 0.0, (0, 0), 0
+    for k in range(4):
+        g.step_time()
+        u = g.get("u")
+        U = u[: (nx - 1) * ny].reshape(ny, nx - 1)
+        V = u[(nx - 1) * ny:].reshape(ny - 1, nx)
+        c = dt * (np.abs(U[: ny - 1, : nx - 1]) / dx[None, : nx - 1] + np.abs(V[:, : nx - 1]) / dy[: ny - 1, None])
+        m = c.max()
+        if m > best:
+            best = m
+            J, I = np.unravel_index(np.argmax(c), c.shape)
+            cell, step = (int(I), int(J)), k
+        gi = g.info()
+        assert abs(gi["cfl_max"] - best) <= 1e-14 * best, (k, gi["cfl_max"], best)
+        assert (gi["cfl_I"], gi["cfl_J"], gi["cfl_ts"]) == (cell[0], cell[1], step), k
+    t = g.timers()
+    for e in ("initialise", "Intermediate Velocity Setup", "Intermediate Velocity Solve", "Poisson Setup",
+              "Poisson Solve", "Velocity Projection", "CFL"):
+        assert e in t and t[e] >= 0, e
