@@ -99,7 +99,8 @@ int cuibm_initialise(cuibm_solver *s) {
     return guard([&] {
         need(s);
         s->s->initialise();
-        if (s->opt.output) writeInfoFile(s->db, s->dom);
+        if (s->opt.output)
+            writeInfoFile(s->db, s->dom, s->opt.outputFolder.empty() ? s->db.caseFolder : s->opt.outputFolder);
         return 0;
     });
 }

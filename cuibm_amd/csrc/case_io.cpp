@@ -337,8 +337,8 @@ void writeData(const std::string &caseFolder, int n, const std::vector<double> &
 }
 
 // src/io/io.cu:406-420
-void writeInfoFile(const ParameterDB &DB, const Domain &D) {
-    std::ofstream f(DB.caseFolder + "/run.info");
+void writeInfoFile(const ParameterDB &DB, const Domain &D, const std::string &folder) {
+    std::ofstream f(folder + "/run.info");
     f << std::setw(20) << "--nx" << "\t" << D.nx << '\n';
     f << std::setw(20) << "--ny" << "\t" << D.ny << '\n';
     f << std::setw(20) << "--startStep" << "\t" << DB.startStep << '\n';

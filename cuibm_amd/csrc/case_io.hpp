@@ -75,7 +75,7 @@ void circlePoints(double cx, double cy, double R, int numPoints, bool selfSize, 
 void makeDirectory(const std::string &path);
 void writeGrid(const std::string &caseFolder, const Domain &D);
 void writeData(const std::string &caseFolder, int n, const std::vector<double> &q, const std::vector<double> &lambda);
-void writeInfoFile(const ParameterDB &DB, const Domain &D);
+void writeInfoFile(const ParameterDB &DB, const Domain &D, const std::string &folder);
 void writeBodies(const std::string &caseFolder, int timeStep, const std::vector<double> &bx, const std::vector<double> &by);
 
 const char *solverTypeName(SolverType t);

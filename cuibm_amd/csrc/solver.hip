@@ -605,6 +605,8 @@ void luo_base::initialise() {
     tagPoints();
     if (opt.output) {
         forceFile.open(outFolder() + "/forces");
+        forceFullFile.open(outFolder() + "/forces_full");
+        forceFullFile << std::setprecision(17);
         midPositionFile.open(outFolder() + "/midPosition");
     }
 }
@@ -849,6 +851,7 @@ void luo_base::writeData() {
     readForces();
     if (timeStep == 1) forceFile << "timestep\tFx\tFy\n";
     forceFile << timeStep * dt << '\t' << forceX << '\t' << forceY << std::endl;
+    forceFullFile << timeStep * dt << '\t' << forceX << '\t' << forceY << '\n';
 }
 
 // luo_base.cu:180-206
@@ -863,6 +866,7 @@ void luo_base::writeCommon() {
 void luo_base::shutDown() {
     NavierStokesSolver::shutDown();
     if (forceFile.is_open()) forceFile.close();
+    if (forceFullFile.is_open()) forceFullFile.close();
     if (midPositionFile.is_open()) midPositionFile.close();
 }
 

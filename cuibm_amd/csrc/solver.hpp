@@ -165,6 +165,7 @@ class luo_base : public NavierStokesSolver {
     DArray<int> solid;
     int nsolid = 0;
     std::ofstream forceFile, midPositionFile;
+    std::ofstream forceFullFile;  // the forces at 17 significant digits (no reference counterpart)
     bool tagged = false;
 
     void boundingBox();
