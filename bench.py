@@ -185,7 +185,7 @@ def main():
         log(f"[bench] cpu: {cpu['value']:.3f} steps/s")
 
     k = mixed["kernel"]
-    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
     line = {
         "metric": METRIC,
         "value": round(mixed["steps_per_s"], 4),

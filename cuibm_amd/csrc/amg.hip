@@ -30,9 +30,13 @@ int grid_for(long n) {
 
 template <class V>
 __global__ void k_jacobi_first(int n, const V *__restrict__ dinv, const double *__restrict__ b,
-                               double *__restrict__ x, double omega) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        x[i] = omega * (double)dinv[i] * b[i];
+                               double *__restrict__ x, double omega, const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first loads
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = omega * (double)dinv[i] * b[i];
+        if (stopv != 0.0) return;
+        x[i] = v;
+    }
 }
 
 // One CSR row product per group of LPR lanes (LPR = 1: a thread per row), with the
@@ -49,7 +53,8 @@ template <class V, int LPR, int OP>
 __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                             const V *__restrict__ v, const double *__restrict__ x,
                                             const double *__restrict__ b, const V *__restrict__ dinv,
-                                            double *__restrict__ y, double omega) {
+                                            double *__restrict__ y, double omega, const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
@@ -59,6 +64,7 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
+        if (stopv != 0.0) return;
         if (lane == 0) {
             if (OP == OP_SPMV) y[i] = s;
             else if (OP == OP_RESID) y[i] = b[i] - s;
@@ -75,7 +81,9 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
 template <class V, int LPR>
 __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                              const V *__restrict__ v, const V *__restrict__ dinv,
-                                             const double *__restrict__ b, double *__restrict__ x2, double omega) {
+                                             const double *__restrict__ b, double *__restrict__ x2, double omega,
+                                             const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
@@ -88,6 +96,7 @@ __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, 
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
+        if (stopv != 0.0) return;
         if (lane == 0) {
             const double wd = omega * (double)dinv[i];
             x2[i] = wd * b[i] + wd * (b[i] - s);
@@ -96,16 +105,17 @@ __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, 
 }
 
 template <class V>
-void launch_pre2(int lpr, const DCsr<V> &A, const V *dinv, const double *b, double *x2, double omega, hipStream_t s) {
+void launch_pre2(int lpr, const DCsr<V> &A, const V *dinv, const double *b, double *x2, double omega, hipStream_t s,
+                 const double *stop) {
     const int n = A.n;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_pre2<V, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
-        case 4: hipLaunchKernelGGL((k_pre2<V, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
-        case 8: hipLaunchKernelGGL((k_pre2<V, 8>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
-        case 16: hipLaunchKernelGGL((k_pre2<V, 16>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
-        default: hipLaunchKernelGGL((k_pre2<V, 32>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega); break;
+        case 1: hipLaunchKernelGGL((k_pre2<V, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 4: hipLaunchKernelGGL((k_pre2<V, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 8: hipLaunchKernelGGL((k_pre2<V, 8>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 16: hipLaunchKernelGGL((k_pre2<V, 16>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        default: hipLaunchKernelGGL((k_pre2<V, 32>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -122,17 +132,17 @@ int lanes_for(const HCsr &A) {
 
 template <class V, int OP>
 void launch_csr_op(int lpr, const DCsr<V> &A, const double *x, const double *b, const V *dinv, double *y, double omega,
-                   hipStream_t s) {
+                   hipStream_t s, const double *stop = nullptr) {
     const int n = A.n;
     if (n <= 0) return;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_csr<V, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
-        case 4: hipLaunchKernelGGL((k_csr<V, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
-        case 8: hipLaunchKernelGGL((k_csr<V, 8, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
-        case 16: hipLaunchKernelGGL((k_csr<V, 16, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
-        default: hipLaunchKernelGGL((k_csr<V, 32, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega); break;
+        case 1: hipLaunchKernelGGL((k_csr<V, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 4: hipLaunchKernelGGL((k_csr<V, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 8: hipLaunchKernelGGL((k_csr<V, 8, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 16: hipLaunchKernelGGL((k_csr<V, 16, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        default: hipLaunchKernelGGL((k_csr<V, 32, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -143,9 +153,11 @@ void launch_csr_op(int lpr, const DCsr<V> &A, const double *x, const double *b, 
 // tree across the wavefront
 template <class V>
 __global__ void __launch_bounds__(256) k_dense(int n, const V *__restrict__ Ainv, const double *__restrict__ b,
-                                               double *__restrict__ x) {
+                                               double *__restrict__ x, const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested before the barrier
     extern __shared__ double sb[];
     for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = b[j];
+    if (stopv != 0.0) return;  // uniform across the block
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -186,6 +198,7 @@ struct AmgDev {
     bool use_fine = false;  // finest level in stencil form (fused kernels)
     P5Dev fine{};
     KernelProbe *probe = nullptr;  // events around the finest-level pre-smoothing launches
+    const double *stop = nullptr;  // the Krylov solver's stop flag (every kernel checks it)
 
     void build(const std::vector<HostAmgLevel> &h, hipStream_t s) {
         lv.resize(h.size());
@@ -232,7 +245,8 @@ struct AmgDev {
         const int n = L.n;
         const double w = prm.omega;
         if (l + 1 == lv.size()) {
-            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), n * sizeof(double), s, n, coarse_inv.p(), b, x);
+            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), n * sizeof(double), s, n, coarse_inv.p(), b, x,
+                               stop);
             CUIBM_LAUNCH_CHECK();
             return;
         }
@@ -243,9 +257,9 @@ struct AmgDev {
             launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s);
             if (probe) probe->mark(s);
             AmgLevelDev<V> &C = lv[1];
-            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
+            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
             vcycle(1, C.b.p(), C.x.p(), s);
-            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s);
+            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s, stop);
             launch_p5_sweep2(fine, w, L.xt.p(), b, x, s);
             return;
         }
@@ -253,13 +267,13 @@ struct AmgDev {
             // x2 -> xt, r -> r, restriction, coarse cycle, prolongation into xt,
             // post-sweeps xt -> r -> x
             AmgLevelDev<V> &C = lv[l + 1];
-            launch_pre2<V>(L.lanesA, L.A, L.dinv.p(), b, L.xt.p(), w, s);
-            launch_csr_op<V, OP_RESID>(L.lanesA, L.A, L.xt.p(), b, nullptr, L.r.p(), 0.0, s);
-            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
+            launch_pre2<V>(L.lanesA, L.A, L.dinv.p(), b, L.xt.p(), w, s, stop);
+            launch_csr_op<V, OP_RESID>(L.lanesA, L.A, L.xt.p(), b, nullptr, L.r.p(), 0.0, s, stop);
+            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
             vcycle(l + 1, C.b.p(), C.x.p(), s);
-            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s);
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), b, L.dinv.p(), L.r.p(), w, s);
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.r.p(), b, L.dinv.p(), x, w, s);
+            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s, stop);
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), b, L.dinv.p(), L.r.p(), w, s, stop);
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.r.p(), b, L.dinv.p(), x, w, s, stop);
             return;
         }
         const int g = grid_for(n);
@@ -267,19 +281,19 @@ struct AmgDev {
         const int swaps = (prm.npre - 1) + prm.npost;
         double *cur = (swaps % 2 == 0) ? x : L.xt.p();
         double *oth = (cur == x) ? L.xt.p() : x;
-        hipLaunchKernelGGL(k_jacobi_first<V>, dim3(g), dim3(BS), 0, s, n, L.dinv.p(), b, cur, w);
+        hipLaunchKernelGGL(k_jacobi_first<V>, dim3(g), dim3(BS), 0, s, n, L.dinv.p(), b, cur, w, stop);
         CUIBM_LAUNCH_CHECK();
         for (int k = 1; k < prm.npre; k++) {
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s);
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s, stop);
             std::swap(cur, oth);
         }
-        launch_csr_op<V, OP_RESID>(L.lanesA, L.A, cur, b, nullptr, L.r.p(), 0.0, s);
+        launch_csr_op<V, OP_RESID>(L.lanesA, L.A, cur, b, nullptr, L.r.p(), 0.0, s, stop);
         AmgLevelDev<V> &C = lv[l + 1];
-        launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s);
+        launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
         vcycle(l + 1, C.b.p(), C.x.p(), s);
-        launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, cur, 0.0, s);
+        launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, cur, 0.0, s, stop);
         for (int k = 0; k < prm.npost; k++) {
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s);
+            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s, stop);
             std::swap(cur, oth);
         }
     }
@@ -324,6 +338,14 @@ AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P
 }
 
 AmgPrecond::~AmgPrecond() = default;
+
+void AmgPrecond::set_stop(const double *stop) {
+    stop_ = stop;
+    im_->d64.stop = stop;
+    im_->d32.stop = stop;
+    im_->d64.fine.stop = stop;
+    im_->d32.fine.stop = stop;
+}
 
 void AmgPrecond::apply(const double *r, double *z, hipStream_t s) {
     if (prm_.precision == 32) im_->d32.vcycle(0, r, z, s);

@@ -36,8 +36,13 @@ template <class T>
 __global__ void k_fill(T *x, T v, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
 }
-__global__ void k_copy(double *__restrict__ y, const double *__restrict__ x, long n) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
+__global__ void k_copy(double *__restrict__ y, const double *__restrict__ x, long n, const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first load
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = x[i];
+        if (stopv != 0.0) return;
+        y[i] = v;
+    }
 }
 __global__ void k_to_float(float *__restrict__ y, const double *__restrict__ x, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = (float)x[i];
@@ -61,9 +66,9 @@ void launch_fill_f(float *x, float v, long n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(BS), 0, s, x, v, n);
     CUIBM_LAUNCH_CHECK();
 }
-void launch_copy(double *y, const double *x, long n, hipStream_t s) {
+void launch_copy(double *y, const double *x, long n, hipStream_t s, const double *stop) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(BS), 0, s, y, x, n);
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(BS), 0, s, y, x, n, stop);
     CUIBM_LAUNCH_CHECK();
 }
 void launch_to_float(float *y, const double *x, long n, hipStream_t s) {
@@ -107,14 +112,21 @@ template void launch_spmv<float, double>(const DCsr<float> &, const double *, do
 __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                                  const double *__restrict__ v, const double *__restrict__ x,
                                                  double *__restrict__ y, const double *__restrict__ a, int want_yy,
-                                                 double *__restrict__ part) {
+                                                 double *__restrict__ part, const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row
     double acc[2] = {0, 0};
+    bool tested = false;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double yi = row_dot(rp, ci, v, x, i);
+        if (!tested) {
+            if (stopv != 0.0) return;  // every thread returns here or below: no barrier is reached
+            tested = true;
+        }
         y[i] = yi;
         acc[0] += yi * a[i];
         acc[1] += yi * yi;
     }
+    if (stopv != 0.0) return;
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = acc[0];
@@ -123,9 +135,9 @@ __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int *__restrict__ 
 }
 
 void launch_spmv_dot(const DCsr<double> &A, const double *x, double *y, const double *a, bool want_yy, double *part,
-                     hipStream_t s) {
+                     hipStream_t s, const double *stop) {
     hipLaunchKernelGGL(k_spmv_dot, dim3(RED_GRID), dim3(BS), 0, s, A.n, A.rp.p(), A.ci.p(), A.v.p(), x, y, a,
-                       (int)want_yy, part);
+                       (int)want_yy, part, stop);
     CUIBM_LAUNCH_CHECK();
 }
 
@@ -163,13 +175,19 @@ void launch_bicg_init(const DCsr<double> &A, const double *x, const double *b, d
 __global__ void __launch_bounds__(BS) k_bicg_s(int n, const double *__restrict__ r, const double *__restrict__ AMp,
                                                double *__restrict__ s, const double *__restrict__ sc,
                                                double *__restrict__ part) {
-    const double alpha = sc[SC_ALPHA];
+    const double stopv = sc[SC_STOP], alpha = sc[SC_ALPHA];
     double acc[1] = {0};
+    bool tested = false;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double si = r[i] - alpha * AMp[i];
+        if (!tested) {
+            if (stopv != 0.0) return;  // every thread returns here or below: no barrier is reached
+            tested = true;
+        }
         s[i] = si;
         acc[0] += si * si;
     }
+    if (stopv != 0.0) return;
     block_sum<1>(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
@@ -180,32 +198,36 @@ void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const 
     CUIBM_LAUNCH_CHECK();
 }
 
-// x += alpha Mp (early exit at the half step)
-__global__ void k_bicg_xhalf(int n, double *__restrict__ x, const double *__restrict__ Mp, const double *__restrict__ sc) {
-    const double alpha = sc[SC_ALPHA];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + alpha * Mp[i];
-}
-
-void launch_bicg_xhalf(int n, double *x, const double *Mp, const double *sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_xhalf, dim3(grid_for(n)), dim3(BS), 0, s, n, x, Mp, sc);
-    CUIBM_LAUNCH_CHECK();
-}
-
-// x += alpha Mp + omega Ms; r = s - omega AMs; partials of (rs.r) and (r.r)
+// x += alpha Mp + omega Ms; r = s - omega AMs; partials of (rs.r) and (r.r). In the
+// iteration k that ended at its half step (SC_STOP = STOP_HALF, SC_HALFK = k) it applies
+// the half-step update x += alpha Mp alone.
 __global__ void __launch_bounds__(BS) k_bicg_xr(int n, double *__restrict__ x, const double *__restrict__ Mp,
                                                 const double *__restrict__ Ms, double *__restrict__ r,
                                                 const double *__restrict__ s, const double *__restrict__ AMs,
                                                 const double *__restrict__ rs, const double *__restrict__ sc,
-                                                double *__restrict__ part) {
-    const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
+                                                double *__restrict__ part, int k) {
+    const double stopv = sc[SC_STOP], alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
+    const bool half = stopv == STOP_HALF && sc[SC_HALFK] == (double)k;
     double acc[2] = {0, 0};
+    bool tested = false;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        x[i] = x[i] + alpha * Mp[i] + omega * Ms[i];
+        const double xh = x[i] + alpha * Mp[i];
+        const double xi = xh + omega * Ms[i];
         const double ri = s[i] - omega * AMs[i];
+        if (!tested) {
+            if (stopv != 0.0 && !half) return;  // every thread returns here or below: no barrier is reached
+            tested = true;
+        }
+        if (half) {
+            x[i] = xh;
+            continue;
+        }
+        x[i] = xi;
         r[i] = ri;
         acc[0] += rs[i] * ri;
         acc[1] += ri * ri;
     }
+    if (stopv != 0.0) return;  // includes the half-step update
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = acc[0];
@@ -214,17 +236,20 @@ __global__ void __launch_bounds__(BS) k_bicg_xr(int n, double *__restrict__ x, c
 }
 
 void launch_bicg_xr(int n, double *x, const double *Mp, const double *Ms, double *r, const double *s_,
-                    const double *AMs, const double *rs, const double *sc, double *part, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_xr, dim3(RED_GRID), dim3(BS), 0, s, n, x, Mp, Ms, r, s_, AMs, rs, sc, part);
+                    const double *AMs, const double *rs, const double *sc, double *part, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_xr, dim3(RED_GRID), dim3(BS), 0, s, n, x, Mp, Ms, r, s_, AMs, rs, sc, part, k);
     CUIBM_LAUNCH_CHECK();
 }
 
 // p = r + beta p - beta omega AMp
 __global__ void k_bicg_p(int n, double *__restrict__ p, const double *__restrict__ r, const double *__restrict__ AMp,
                          const double *__restrict__ sc) {
-    const double beta = sc[SC_BETA], omega = sc[SC_OMEGA];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        p[i] = r[i] + beta * p[i] - beta * omega * AMp[i];
+    const double stopv = sc[SC_STOP], beta = sc[SC_BETA], omega = sc[SC_OMEGA];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = r[i] + beta * p[i] - beta * omega * AMp[i];
+        if (stopv != 0.0) return;
+        p[i] = v;
+    }
 }
 
 void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s) {
@@ -232,8 +257,12 @@ void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const d
     CUIBM_LAUNCH_CHECK();
 }
 
-// sum the partial groups and update the recurrence scalars (one block)
-__global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__ part, int op, double *__restrict__ sc) {
+// sum the partial groups and update the recurrence scalars (one block); FIN_SS and
+// FIN_BETA then apply the host's stopping tests (cusp::monitor on ||s|| and ||r||, the
+// breakdown tests, the iteration limit) and record the outcome in SC_STOP
+__global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__ part, int op, double *__restrict__ sc,
+                                                      double tol, int maxit) {
+    const double stopv = (op != FIN_INIT && op != FIN_DOT2) ? sc[SC_STOP] : 0.0;  // tested after the loads
     __shared__ double sh[2][16];
     double v[2] = {0, 0};
     const int ng = (op == FIN_ALPHA || op == FIN_SS) ? 1 : 2;
@@ -242,6 +271,7 @@ __global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__
         for (int k = threadIdx.x; k < RED_GRID; k += blockDim.x) t += part[g * RED_GRID + k];
         v[g] = t;
     }
+    if (stopv != 0.0) return;  // uniform across the block, before the barrier
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int g = 0; g < ng; g++) {
         double t = wave_sum(v[g]);
@@ -256,16 +286,23 @@ __global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__
         tot[g] = t;
     }
     switch (op) {
-        case FIN_INIT:  // (r.r, b.b)
+        case FIN_INIT:  // (r.r, b.b); a (re)start clears the stop code, the count goes on
             sc[SC_RR] = tot[0];
             sc[SC_RHO] = tot[0];
             sc[SC_BB] = tot[1];
+            sc[SC_STOP] = 0.0;
             break;
         case FIN_ALPHA:  // rs.AMp
             sc[SC_ALPHA] = sc[SC_RHO] / tot[0];
             break;
         case FIN_SS:
             sc[SC_SS] = tot[0];
+            if (!isfinite(tot[0]) || !isfinite(sc[SC_ALPHA])) {
+                sc[SC_STOP] = STOP_BREAK_HALF;
+            } else if (sqrt(tot[0]) <= tol) {
+                sc[SC_STOP] = STOP_HALF;
+                sc[SC_HALFK] = sc[SC_ITS];
+            }
             break;
         case FIN_OMEGA: {  // (AMs.s, AMs.AMs); a breakdown leaves omega = 0 so that x stays finite
             const double w = tot[0] / tot[1];
@@ -276,6 +313,14 @@ __global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__
             sc[SC_BETA] = (tot[0] / sc[SC_RHO]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
             sc[SC_RHO] = tot[0];
             sc[SC_RR] = tot[1];
+            sc[SC_ITS] = sc[SC_ITS] + 1.0;
+            const double rn = sqrt(tot[1]);
+            if (rn <= tol)
+                sc[SC_STOP] = STOP_CONVERGED;
+            else if (sc[SC_OMEGA] == 0.0 || sc[SC_RHO] == 0.0 || !isfinite(sc[SC_BETA]) || !isfinite(rn))
+                sc[SC_STOP] = STOP_BREAK_FULL;
+            else if (sc[SC_ITS] >= (double)maxit)
+                sc[SC_STOP] = STOP_MAXIT;
             break;
         }
         case FIN_DOT2:
@@ -285,8 +330,8 @@ __global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__
     }
 }
 
-void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, part, op, sc);
+void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s, double tol, int maxit) {
+    hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, part, op, sc, tol, maxit);
     CUIBM_LAUNCH_CHECK();
 }
 
@@ -312,12 +357,17 @@ void launch_dot2(long n, const double *x, const double *y, double *part, hipStre
 
 // z = dinv * r (Jacobi)
 __global__ void k_jacobi_apply(int n, const double *__restrict__ dinv, const double *__restrict__ r,
-                               double *__restrict__ z) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = dinv[i] * r[i];
+                               double *__restrict__ z, const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first loads
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = dinv[i] * r[i];
+        if (stopv != 0.0) return;
+        z[i] = v;
+    }
 }
 
-void launch_jacobi_apply(int n, const double *dinv, const double *r, double *z, hipStream_t s) {
-    hipLaunchKernelGGL(k_jacobi_apply, dim3(grid_for(n)), dim3(BS), 0, s, n, dinv, r, z);
+void launch_jacobi_apply(int n, const double *dinv, const double *r, double *z, hipStream_t s, const double *stop) {
+    hipLaunchKernelGGL(k_jacobi_apply, dim3(grid_for(n)), dim3(BS), 0, s, n, dinv, r, z, stop);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -83,15 +83,22 @@ __global__ void k_p5_coef(int nx, int ny, const double *__restrict__ dx, const d
 __global__ void __launch_bounds__(BS) k_p5_spmv_dot(P5Dev A, const double *__restrict__ x, double *__restrict__ y,
                                                     const double *__restrict__ a, int want_yy,
                                                     double *__restrict__ part) {
+    const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
     const int n = A.nx * A.ny, blk = xcd_band(RED_GRID);
     double acc[2] = {0, 0};
+    bool tested = false;
     for (int ip = blk * BS + threadIdx.x; ip < n; ip += RED_GRID * BS) {
         const int I = ip % A.nx, J = ip / A.nx;
         const double yi = p5_row(A, I, J, p5_diag(A, I, J, ip), [&](int o) { return x[ip + o]; }, A.nx);
+        if (!tested) {
+            if (stopv != 0.0) return;  // every thread returns here or below: no barrier is reached
+            tested = true;
+        }
         if (y) y[ip] = yi;
         acc[0] += yi * a[ip];
         acc[1] += yi * yi;
     }
+    if (stopv != 0.0) return;
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         part[blk] = acc[0];
@@ -182,6 +189,7 @@ constexpr int N2 = H2 * W2, N1 = H1 * W1, R2 = (N2 + 255) / 256, R1 = (N1 + 255)
 __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double *__restrict__ b,
                                                 double *__restrict__ x2g, double *__restrict__ rg, int tiles_x,
                                                 int ntiles) {
+    const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
     __shared__ TileCoef cf;
     __shared__ double sb[N2], swd[N2], sx2[N1];
     const int t = xcd_band(ntiles);
@@ -198,6 +206,7 @@ __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double 
         bv[m] = in ? b[ip] : 0.0;
         dv[m] = in ? A.dinv[ip] : 0.0;
     }
+    if (stopv != 0.0) return;  // uniform across the block, before the first barrier
 #pragma unroll
     for (int m = 0; m < R2; m++) {
         const int k = threadIdx.x + 256 * m;
@@ -237,6 +246,7 @@ __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double 
 __global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, double w, const double *__restrict__ x,
                                                    const double *__restrict__ b, double *__restrict__ xo, int tiles_x,
                                                    int ntiles) {
+    const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
     __shared__ TileCoef cf;
     __shared__ double sx[N2], sb[N1], swd[N1];
     const int t = xcd_band(ntiles);
@@ -260,6 +270,7 @@ __global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, double w, const doub
         bv[m] = in ? b[ip] : 0.0;
         dv[m] = in ? A.dinv[ip] : 0.0;
     }
+    if (stopv != 0.0) return;  // uniform across the block, before the first barrier
 #pragma unroll
     for (int m = 0; m < R2; m++) {
         const int k = threadIdx.x + 256 * m;

@@ -25,7 +25,9 @@ JacobiPrecond::JacobiPrecond(const DCsr<double> &A, hipStream_t s) : n_(A.n) {
     launch_diag_inv(A, dinv_.p(), s);
 }
 
-void JacobiPrecond::apply(const double *r, double *z, hipStream_t s) { launch_jacobi_apply(n_, dinv_.p(), r, z, s); }
+void JacobiPrecond::apply(const double *r, double *z, hipStream_t s) {
+    launch_jacobi_apply(n_, dinv_.p(), r, z, s, stop_);
+}
 
 Bicgstab::Bicgstab(int n) : n_(n) {
     for (auto *a : {&r_, &rs_, &p_, &Mp_, &AMp_, &s_, &Ms_, &AMs_}) a->alloc(n);
@@ -43,61 +45,83 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
     if (A.n() != n_) throw Error("BiCGSTAB workspace size mismatch", ERR_ARGUMENT);
     const int n = n_;
     double *sc = sc_.p(), *part = part_.p();
-    auto read = [&](int slot) {
-        CUIBM_HIP(hipMemcpyAsync(host_sc_ + slot, sc + slot, sizeof(double), hipMemcpyDeviceToHost, s));
-        CUIBM_HIP(hipStreamSynchronize(s));
-        return host_sc_[slot];
-    };
+    const double *stop = sc + SC_STOP;
+    A.set_stop(stop);
+    M.set_stop(stop);
     SolveStat st;
-    auto fetch = [&]() {  // the whole scalar block: one small copy, one sync
+    auto fetch = [&]() {  // the scalar block: one small copy, one sync
         CUIBM_HIP(hipMemcpyAsync(host_sc_, sc, SC_D0 * sizeof(double), hipMemcpyDeviceToHost, s));
         CUIBM_HIP(hipStreamSynchronize(s));
     };
-    auto start = [&]() {  // r = b - A x, rs = p = r
+    auto start = [&]() {  // r = b - A x, rs = p = r; clears the stop code
         A.init(x, b, r_.p(), p_.p(), rs_.p(), part, s);
         launch_bicg_finish(part, FIN_INIT, sc, s);
         fetch();
     };
+    launch_fill(sc + SC_ITS, 0.0, 1, s);
     start();
     const double tol = rtol * std::sqrt(host_sc_[SC_BB]);
     double rn = std::sqrt(host_sc_[SC_RR]);
+    const double rn0 = rn;
     int it = 0, restarts = 0;
     st.tolerance = tol;
-    while (!(rn <= tol) && it < maxit) {
+    // the first half of iteration k (through the test on ||s||), then the second half
+    auto half1 = [&]() {
         M.apply(p_.p(), Mp_.p(), s);
         A.spmv_dot(Mp_.p(), AMp_.p(), rs_.p(), false, part, s);
         launch_bicg_finish(part, FIN_ALPHA, sc, s);
         launch_bicg_s(n, r_.p(), AMp_.p(), s_.p(), sc, part, s);
-        launch_bicg_finish(part, FIN_SS, sc, s);
-        fetch();
-        const double ss = host_sc_[SC_SS];
-        if (!std::isfinite(ss) || !std::isfinite(host_sc_[SC_ALPHA])) {
-            if (++restarts > kMaxRestarts) break;
-            start();  // x is untouched in this half-iteration
-            rn = std::sqrt(host_sc_[SC_RR]);
-            continue;
-        }
-        rn = std::sqrt(ss);
-        if (rn <= tol) {
-            launch_bicg_xhalf(n, x, Mp_.p(), sc, s);
-            break;
-        }
+        launch_bicg_finish(part, FIN_SS, sc, s, tol, maxit);
+    };
+    auto half2 = [&](int k) {
         M.apply(s_.p(), Ms_.p(), s);
         A.spmv_dot(Ms_.p(), AMs_.p(), s_.p(), true, part, s);
         launch_bicg_finish(part, FIN_OMEGA, sc, s);
-        launch_bicg_xr(n, x, Mp_.p(), Ms_.p(), r_.p(), s_.p(), AMs_.p(), rs_.p(), sc, part, s);
-        launch_bicg_finish(part, FIN_BETA, sc, s);
+        launch_bicg_xr(n, x, Mp_.p(), Ms_.p(), r_.p(), s_.p(), AMs_.p(), rs_.p(), sc, part, k, s);
+        launch_bicg_finish(part, FIN_BETA, sc, s, tol, maxit);
         launch_bicg_p(n, p_.p(), r_.p(), AMp_.p(), sc, s);
-        it++;
+    };
+    while (!(rn <= tol) && it < maxit) {
+        // iterations to queue before the next read-back: a fraction of the estimate from
+        // the residual reduction so far (or of the last solve), so that few launches
+        // fall past the end; one in exact mode
+        int batch = 1;
+        if (!exact_) {
+            double rate = rate_;
+            if (it > 0 && rn < rn0) rate = std::pow(rn / rn0, 1.0 / it);
+            if (rate > 0 && rate < 1) {
+                const double left = std::log(tol / rn) / std::log(rate);
+                batch = (int)std::max(1.0, std::min(16.0, std::floor(0.7 * left)));
+            }
+            batch = std::min(batch, maxit - it);
+        }
+        for (int j = 0; j < batch; j++) {
+            half1();
+            if (exact_) {  // no launch past a half-step exit or breakdown
+                fetch();
+                if (host_sc_[SC_STOP] == STOP_HALF)  // the half-step update alone
+                    launch_bicg_xr(n, x, Mp_.p(), Ms_.p(), r_.p(), s_.p(), AMs_.p(), rs_.p(), sc, part, it + j, s);
+                if (host_sc_[SC_STOP] != 0.0) break;
+            }
+            half2(it + j);
+        }
         fetch();
+        it = (int)host_sc_[SC_ITS];
+        const int code = (int)host_sc_[SC_STOP];
+        if (code == STOP_HALF) {
+            rn = std::sqrt(host_sc_[SC_SS]);
+            break;
+        }
         rn = std::sqrt(host_sc_[SC_RR]);
-        if (!(rn <= tol) && (host_sc_[SC_OMEGA] == 0.0 || host_sc_[SC_RHO] == 0.0 ||
-                             !std::isfinite(host_sc_[SC_BETA]) || !std::isfinite(rn))) {
+        if (code == STOP_BREAK_HALF || code == STOP_BREAK_FULL) {
             if (++restarts > kMaxRestarts) break;
-            start();
+            start();  // x is untouched by a half-step breakdown
             rn = std::sqrt(host_sc_[SC_RR]);
         }
     }
+    if (it > 0 && rn > 0 && rn0 > 0 && rn < rn0) rate_ = std::pow(rn / rn0, 1.0 / it);
+    A.set_stop(nullptr);
+    M.set_stop(nullptr);
     st.restarts = restarts;
     st.iterations = it;
     st.residual = rn;

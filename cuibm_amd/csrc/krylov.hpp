@@ -11,31 +11,39 @@ namespace cuibm {
 
 constexpr int RED_GRID = 2048;  // blocks of every reducing kernel
 
-// slots of the device scalar block
-enum { SC_BB = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_BETA, SC_SS, SC_RR, SC_D0, SC_D1, SC_COUNT = 16 };
+// slots of the device scalar block. SC_STOP is the device-side end of the iteration:
+// 0 running, STOP_* below otherwise; every kernel of the iteration returns at once when
+// it is set, so the host may queue several iterations before reading the block back.
+enum { SC_BB = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_BETA, SC_SS, SC_RR, SC_STOP, SC_ITS, SC_HALFK, SC_D0, SC_D1,
+       SC_COUNT = 16 };
+enum { STOP_CONVERGED = 1, STOP_HALF = 2, STOP_BREAK_HALF = 3, STOP_BREAK_FULL = 4, STOP_MAXIT = 5 };
 // finish-kernel operations
 enum { FIN_INIT = 0, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_BETA, FIN_DOT2 };
 
 // ---- k_blas.hip
 void launch_fill(double *x, double v, long n, hipStream_t s);
 void launch_fill_f(float *x, float v, long n, hipStream_t s);
-void launch_copy(double *y, const double *x, long n, hipStream_t s);
+void launch_copy(double *y, const double *x, long n, hipStream_t s, const double *stop = nullptr);
 void launch_to_float(float *y, const double *x, long n, hipStream_t s);
 void launch_to_double(double *y, const float *x, long n, hipStream_t s);
 template <class V, class X>
 void launch_spmv(const DCsr<V> &A, const X *x, X *y, hipStream_t s);
 void launch_spmv_dot(const DCsr<double> &A, const double *x, double *y, const double *a, bool want_yy, double *part,
-                     hipStream_t s);
+                     hipStream_t s, const double *stop = nullptr);
 void launch_bicg_init(const DCsr<double> &A, const double *x, const double *b, double *r, double *p, double *rs,
                       double *part, hipStream_t s);
 void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const double *sc, double *part, hipStream_t s);
-void launch_bicg_xhalf(int n, double *x, const double *Mp, const double *sc, hipStream_t s);
+// x += alpha Mp + omega Ms, r = s - omega AMs with partials; in iteration k after a
+// half-step exit of that iteration, x += alpha Mp only
 void launch_bicg_xr(int n, double *x, const double *Mp, const double *Ms, double *r, const double *s_,
-                    const double *AMs, const double *rs, const double *sc, double *part, hipStream_t s);
+                    const double *AMs, const double *rs, const double *sc, double *part, int k, hipStream_t s);
 void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s);
-void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s);
+// sums the partials and updates the scalars; FIN_SS and FIN_BETA also apply the stopping
+// tests of the monitor (tol) and the iteration limit (maxit) and set SC_STOP
+void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s, double tol = 0, int maxit = 0);
 void launch_dot2(long n, const double *x, const double *y, double *part, hipStream_t s);
-void launch_jacobi_apply(int n, const double *dinv, const double *r, double *z, hipStream_t s);
+void launch_jacobi_apply(int n, const double *dinv, const double *r, double *z, hipStream_t s,
+                         const double *stop = nullptr);
 void launch_diag_inv(const DCsr<double> &A, double *dinv, hipStream_t s);
 
 // ---- linear operators (the A of BiCGSTAB)
@@ -51,6 +59,11 @@ class LinOp {
                       hipStream_t s) const = 0;
     // algorithmic bytes of one product y = A x
     virtual double spmv_bytes() const = 0;
+    // device flag (the solver's SC_STOP slot): the products return at once while it is set
+    void set_stop(const double *stop) const { stop_ = stop; }
+
+  protected:
+    mutable const double *stop_ = nullptr;
 };
 
 class CsrOp : public LinOp {
@@ -59,7 +72,7 @@ class CsrOp : public LinOp {
     int n() const override { return A_.n; }
     void spmv_dot(const double *x, double *y, const double *a, bool want_yy, double *part,
                   hipStream_t s) const override {
-        launch_spmv_dot(A_, x, y, a, want_yy, part, s);
+        launch_spmv_dot(A_, x, y, a, want_yy, part, s, stop_);
     }
     void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
               hipStream_t s) const override {
@@ -79,12 +92,18 @@ class Preconditioner {
     virtual std::string name() const = 0;
     // bytes moved by one apply, counted as distinct arrays touched per kernel
     virtual double bytes_per_apply() const = 0;
+    // device flag (the solver's SC_STOP slot): every kernel of an apply returns at once
+    // while it is set
+    virtual void set_stop(const double *stop) { stop_ = stop; }
+
+  protected:
+    const double *stop_ = nullptr;
 };
 
 class IdentityPrecond : public Preconditioner {
   public:
     explicit IdentityPrecond(int n) : n_(n) {}
-    void apply(const double *r, double *z, hipStream_t s) override { launch_copy(z, r, n_, s); }
+    void apply(const double *r, double *z, hipStream_t s) override { launch_copy(z, r, n_, s, stop_); }
     std::string name() const override { return "NONE"; }
     double bytes_per_apply() const override { return 16.0 * n_; }
 
@@ -163,6 +182,7 @@ struct P5Dev {
     int centre;                        // cell whose diagonal is doubled besides cell 0, or -1
     const double *cE, *cW, *cN, *cS;   // coupling magnitudes; 0 where the neighbour is missing
     const double *dinv;                // 1 / diagonal, per cell
+    const double *stop = nullptr;      // the kernels return at once while *stop != 0
 };
 
 class AmgPrecond : public Preconditioner {
@@ -173,6 +193,7 @@ class AmgPrecond : public Preconditioner {
     AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine = nullptr);
     ~AmgPrecond() override;
     void apply(const double *r, double *z, hipStream_t s) override;
+    void set_stop(const double *stop) override;
     std::string name() const override { return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)" : "SMOOTHED_AGGREGATION"; }
     double bytes_per_apply() const override;
     AmgHierarchyInfo info() const;
@@ -222,10 +243,17 @@ class Bicgstab {
     SolveStat solve(const LinOp &A, double *x, const double *b, Preconditioner &M, double rtol, int maxit,
                     hipStream_t s);
     int n() const { return n_; }
+    // exact launches: read the scalars back at every stopping test (twice per iteration)
+    // and launch nothing past the end; otherwise iterations are queued in batches sized
+    // from the observed convergence rate and the device stops them (kernel timing uses
+    // the exact mode so that no launch returns early)
+    void set_exact(bool on) { exact_ = on; }
 
   private:
     static constexpr int kMaxRestarts = 64;
     int n_;
+    bool exact_ = false;
+    double rate_ = 0;  // mean residual reduction per iteration of the last solve
     DArray<double> r_, rs_, p_, Mp_, AMp_, s_, Ms_, AMs_, part_, sc_;
     double *host_sc_ = nullptr;
 };

@@ -32,7 +32,7 @@ class PoissonStencil : public LinOp {
     void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
               hipStream_t s) const override;
     double spmv_bytes() const override { return 16.0 * n() + 32.0 * (nx_ + ny_); }
-    P5Dev dev() const { return P5Dev{nx_, ny_, centre_, cE_.p(), cW_.p(), cN_.p(), cS_.p(), dinv_.p()}; }
+    P5Dev dev() const { return P5Dev{nx_, ny_, centre_, cE_.p(), cW_.p(), cN_.p(), cS_.p(), dinv_.p(), stop_}; }
     // y = A x without dot partials (tests)
     void apply(const double *x, double *y, hipStream_t s) const;
 
