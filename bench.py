@@ -22,8 +22,9 @@ about 55 steps in this method (GPU and CPU restatement alike); the default windo
 
 Prints one JSON line. ``roofline`` reports the dominant kernel of the step (the
 fused finest-level pre-smoothing + residual kernel of the Poisson V-cycle, one per
-V-cycle and two V-cycles per BiCGSTAB iteration), timed with HIP events around each
-of its launches inside a few further live time steps after the timed window (the
+V-cycle and two V-cycles per BiCGSTAB iteration), timed with a HIP start/stop event
+pair attached to each of its launches (hipExtLaunchKernel, which stamps the kernel's
+own begin and end) inside a few further live time steps after the timed window (the
 timed window itself carries no probes); ``kernels`` lists back-to-back timings of
 single kernels on the live inputs for comparison; ``traffic`` is the
 HBM byte count per launch of that kernel from hardware counters, read from the
@@ -89,7 +90,7 @@ def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5
         out["velocity_bytes_per_step"]
     out["hbm_gbps_step"] = out["step_bytes"] / (out["ms_per_step"] * 1e-3) / 1e9
     if kernel:
-        # roofline kernel: HIP events around each of its launches inside `probe_steps`
+        # roofline kernel: HIP events attached to each of its launches inside `probe_steps`
         # further live time steps (after the timed window, which stays unprobed)
         kms, kbytes = s.time_kernel("live:" + kernel, probe_steps)
         out["kernel"] = {"name": kernel, "ms": kms, "bytes": kbytes, "steps": probe_steps}
@@ -131,6 +132,51 @@ def cpu_baseline(case_dir, steps):
                       f"{dt:.1f} s of CPU work; initialisation excluded"}
 
 
+# The reference's own published run times (scripts/validation/performance.py:22-24 on a Tesla
+# K20; scripts/validation/performance2.py:24-25, hardware not stated): whole runs of the
+# same step, solver tolerance 1e-5, FP64 smoothed aggregation. Case folders restate
+# validation/error/cylinder/{fadlun4,external4,embedded4} and
+# validation/error/oscflow2/{external05,embedded05}.
+PUBLISHED = [
+    ("cylinderRe40_890_fadlun", "modified Fadlun, Re=40 cylinder, 890x890, 1000 steps", 563.0, "Tesla K20"),
+    ("cylinderRe40_890_luo", "Luo external, Re=40 cylinder, 890x890, 1000 steps", 174.0, "Tesla K20"),
+    ("cylinderRe40_462_embedded", "Luo embedded, Re=40 cylinder, 462x462, 1000 steps", 762.0, "Tesla K20"),
+    ("oscFlowRe100_luo", "Luo external, oscillating cylinder in flow, 224x224, 2000 steps", 61.0, "not stated"),
+    ("oscFlowRe100_embedded", "Luo embedded, oscillating cylinder in flow, 224x224, 2000 steps", 581.0,
+     "not stated"),
+]
+
+
+def published_runs(cu):
+    """each published workload run in full (nt steps from the impulsive start), FP64 hierarchy"""
+    rows = []
+    for name, what, ref_s, hw in PUBLISHED:
+        t0 = time.perf_counter()
+        s = cu.Solver(os.path.join(cu.CASES, name), "-amgPrecision", "64", diagnostics=False)
+        s.initialise()
+        t_init = time.perf_counter() - t0
+        nt = int(s.case_scalars()["nt"])
+        cu.device_synchronize()
+        w0 = time.perf_counter()
+        ms, done = 0.0, 0
+        while done < nt:  # in tenths, with a progress line each
+            k = min(max(nt // 10, 1), nt - done)
+            ms += s.time_steps(k)
+            done += k
+            log(f"[bench]   {name}: {done}/{nt} steps, {time.perf_counter() - w0:.1f} s")
+        cu.device_synchronize()
+        wall = time.perf_counter() - w0
+        i1 = s.info()
+        s.close()
+        run_s = t_init + wall
+        rows.append({"case": name, "workload": what, "steps": nt, "init_s": round(t_init, 3),
+                     "steps_s": round(wall, 3), "steps_per_s": round(1e3 * nt / ms, 2), "run_s": round(run_s, 3),
+                     "published_s": ref_s, "published_hardware": hw, "speedup_vs_published": round(ref_s / run_s, 1),
+                     "poisson_iters_per_step": round(i1["iter2_total"] / nt, 2), "cd": 2.0 * i1["forceX"]})
+        log(f"[bench] {name}: {nt} steps in {wall:.2f} s (+{t_init:.2f} s set-up), published {ref_s:.0f} s")
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +192,9 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="initialise the case, run two time steps and the copy calibration kernel "
                          "(for hardware-counter runs)")
+    ap.add_argument("--published", action="store_true",
+                    help="run the reference's published whole-run workloads in full and print their times "
+                         "beside the published ones (one JSON line; not the headline metric)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -157,6 +206,12 @@ def main():
     import cuibm_amd as cu
 
     case = os.path.join(cu.CASES, CASE)
+    if args.published:
+        rows = published_runs(cu)
+        print(json.dumps({"metric": "seconds per published whole run (set-up + all steps, no file output)",
+                          "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+                          "runs": rows}), flush=True)
+        return
     if args.kernels_only:
         # two live time steps (their kernels are the ones the counters should see), then
         # the copy calibration kernel
@@ -223,7 +278,8 @@ def main():
             "rocprof_us_per_launch": read_profile("rocprof_us_per_launch"),
             "bytes_per_launch": k["bytes"],
             "us_per_launch": round(k["ms"] * 1e3, 2),
-            "timing": f"HIP events around each in-step launch over {k['steps']} live time steps after the timed window",
+            "timing": f"HIP start/stop events attached to each in-step launch (hipExtLaunchKernel: the kernel's own "
+                      f"begin and end) over {k['steps']} live time steps after the timed window",
         },
         "kernels": {n: {kk: round(vv, 2) for kk, vv in d.items()} for n, d in mixed["kernels"].items()},
         "cpu_baseline": cpu,

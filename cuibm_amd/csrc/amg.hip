@@ -3,8 +3,8 @@
 //
 // One application is one V-cycle from a zero initial guess: on every level but the
 // coarsest, npre weighted-Jacobi sweeps (omega = 0.8; the first is x = omega D^-1 b),
-// residual, restriction with R = P^T, recursion, prolongation-correction, npost
-// sweeps; the coarsest level (<= 2500 unknowns) applies a dense inverse, one
+// residual, restriction with R = P^T (P^T D^-1 on a row-scaled finest level),
+// recursion, prolongation-correction, npost sweeps; the coarsest level (<= 2500 unknowns) applies a dense inverse, one
 // wavefront per row.
 //
 // Mixed precision stores only the matrix values (A_l, P_l, R_l, D_l^-1 and the
@@ -253,9 +253,7 @@ struct AmgDev {
         if (l == 0 && use_fine && prm.npre == 2 && prm.npost == 2) {
             // stencil finest level: x2 -> xt and r -> r, restriction, coarse cycle,
             // prolongation into xt, both post-sweeps straight into x
-            if (probe) probe->mark(s);
-            launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s);
-            if (probe) probe->mark(s);
+            launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s, probe);
             AmgLevelDev<V> &C = lv[1];
             launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
             vcycle(1, C.b.p(), C.x.p(), s);
@@ -310,7 +308,7 @@ KernelProbe &AmgPrecond::pre_probe() { return im_->probe; }
 
 AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine)
     : im_(new Impl), prm_(p) {
-    im_->host = amg_build(A, p.coarse_max, p.max_levels);
+    im_->host = amg_build(A, p.coarse_max, p.max_levels, p.scale0);
     fine_ = fine != nullptr && im_->host.size() > 1 && p.npre == 2 && p.npost == 2;
     if (fine_) {
         if (fine->nx * fine->ny != A.n) throw Error("stencil and matrix sizes differ", ERR_ARGUMENT);

@@ -9,7 +9,11 @@
 //     (3) a node still free forms an aggregate with its free neighbours;
 //   tentative prolongator: piecewise constant, each column scaled to unit norm;
 //   smoothed prolongator P = (I - 4/(3 rho) D^-1 A) T, rho = max_i sum_j |a_ij/a_ii|;
-//   restriction R = P^T, coarse operator A_c = R A P;
+//   restriction R = P^T, coarse operator A_c = R A P; with scale0 the finest level
+//     restricts with R = P^T D^-1 instead (the hierarchy of the row-scaled D^-1 A, whose
+//     smoothing and prolongator are those of A): a matrix that mixes unit-diagonal
+//     interpolation rows with Poisson rows of diagonal 4 dt/h^2 otherwise gets coarse
+//     diagonals of either sign where the two kinds of row share an aggregate;
 //   coarsening stops at <= coarse_max unknowns (2500), solved with a dense inverse.
 // Everything is sequential and in index order, so the hierarchy is deterministic.
 #include <algorithm>
@@ -118,7 +122,7 @@ int aggregate(const HCsr &A, std::vector<int> &agg) {
 
 }  // namespace
 
-std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_levels) {
+std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_levels, bool scale0) {
     std::vector<HostAmgLevel> lv;
     HCsr A = A0;
     A.ncols = A.n;
@@ -177,6 +181,8 @@ std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_leve
         }
         HostAmgLevel L;
         L.R = transpose(P);
+        if (scale0 && lv.empty())  // column j of P^T scaled by 1/a_jj
+            for (size_t k = 0; k < L.R.v.size(); k++) L.R.v[k] = L.R.v[k] * (1.0 / d[L.R.ci[k]]);
         const HCsr AP = multiply(A, P);
         HCsr Ac = multiply(L.R, AP);
         L.A = A;

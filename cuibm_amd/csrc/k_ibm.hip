@@ -14,6 +14,8 @@
 // 4x4 system into four corner weights, and per-step apply kernels that are pure
 // gathers over compact node lists. Ghost-node velocity updates are staged so that
 // every read sees the field as it was before the update.
+#include <algorithm>
+
 #include "ibm.hpp"
 #include "krylov.hpp"
 
@@ -375,31 +377,53 @@ __global__ void __launch_bounds__(BS) k_tag_p_resolve(GridDev g, BoxDev box, Bod
     }
 }
 
-// zero_pressure_luo / zero_x_luo / zero_y_luo (tagPoints.cu:1157-1298): one thread per row
-__global__ void k_zero_rows(int *t, BoxDev box, int rowlen, int kmax, int off, int nthr0, int nrows, long len) {
-    const int jt = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jt >= nthr0) return;
-    const int J = box.j_start + jt, j_end = box.j_start + box.h;
-    if (J > j_end || J >= nrows) return;
-    const int i_start = box.i_start, i_end = box.i_start + box.w;
+// zero_pressure_luo / zero_x_luo / zero_y_luo (tagPoints.cu:1157-1298): each row of the
+// box is scanned left to right, marking the body interior solid (tag 0). One block per
+// row stages the row segment [i_start-1, max(kmax, i_end-1)] in LDS. The reference's
+// look-ahead loop at a hit i asks whether any position k in (i, kmax] holds a tag other
+// than -1; the scan writes only at positions <= i, so those positions still hold their
+// entry values and the answer is L > i, with L the last such position of the row at entry
+// (a block max). Lane 0 then runs the serial scan on LDS and stores each change; a
+// position past the end of the tag array reads as -1, as in the look-ahead's bound.
+__global__ void __launch_bounds__(BS) k_zero_rows(int *t, BoxDev box, int rowlen, int kmax, int off, long len) {
+    extern __shared__ int seg[];
+    __shared__ int red[BS / 64];
+    const int J = box.j_start + blockIdx.x;
+    const int i0 = box.i_start - 1, i_end = box.i_start + box.w;
+    const int nseg = max(kmax, i_end - 1) - i0 + 1;
+    const long base = (long)J * rowlen + off;
+    int last = i0;
+    for (int k = threadIdx.x; k < nseg; k += blockDim.x) {
+        const long q = base + i0 + k;
+        const int v = (q >= 0 && q < len) ? t[q] : -1;
+        seg[k] = v;
+        const int pos = i0 + k;
+        if (pos > box.i_start && pos <= kmax && v != -1) last = max(last, pos);
+    }
+    for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = last;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int L = red[0];
+    for (int w = 1; w < BS / 64; w++) L = max(L, red[w]);
     bool rowIsntDone = true, flag = false;
-    for (int i = i_start; i < i_end; i++) {
-        const int Ix = J * rowlen + i + off;
-        if (t[Ix - 1] >= 0) {
-            if (t[Ix] != -1 && t[Ix - 1] == 0) rowIsntDone = false;
-            if (t[Ix] == -1 && rowIsntDone) t[Ix] = 0;
-            if (t[Ix] == 0 && t[Ix - 1] != 0) {
-                int k = i;
-                while (k < kmax) {
-                    k++;
-                    const long q = (long)J * rowlen + k + off;
-                    if (q < len && t[q] != -1) flag = true;
-                }
-                if (!flag) {
-                    t[Ix] = -1;
-                    rowIsntDone = false;
-                }
+    for (int i = box.i_start; i < i_end; i++) {
+        const int prev = seg[i - 1 - i0];
+        int cur = seg[i - i0];
+        if (prev < 0) continue;
+        if (cur != -1 && prev == 0) rowIsntDone = false;
+        int nv = cur;
+        if (cur == -1 && rowIsntDone) nv = 0;
+        if (nv == 0 && prev != 0) {
+            flag = flag || L > i;
+            if (!flag) {
+                nv = -1;
+                rowIsntDone = false;
             }
+        }
+        if (nv != cur) {
+            seg[i - i0] = nv;
+            if (base + i >= 0 && base + i < len) t[base + i] = nv;
         }
     }
 }
@@ -874,8 +898,16 @@ void launch_tag_pressure_resolve(const GridDev &g, const BoxDev &box, const Body
 
 void launch_zero_rows(int *tags, const BoxDev &box, int rowlen, int kmax, int off, int nrows, long len,
                       hipStream_t s) {
+    // rows j_start .. j_end (inclusive) below nrows, and no more than the reference's thread
+    // count over the box width
     const int nthr0 = ((int)((box.w - 0.5) / 256) + 1) * 256;
-    hipLaunchKernelGGL(k_zero_rows, dim3(nblk(nthr0)), dim3(BS), 0, s, tags, box, rowlen, kmax, off, nthr0, nrows, len);
+    const int jlast = std::min({box.j_start + box.h, nrows - 1, box.j_start + nthr0 - 1});
+    const int rows = jlast - box.j_start + 1;
+    if (rows <= 0 || box.w <= 0) return;
+    const int nseg = std::max(kmax, box.i_start + box.w - 1) - (box.i_start - 1) + 1;
+    const size_t lds = (size_t)nseg * sizeof(int);
+    if (lds > 65536) throw Error("tagging: grid row of " + std::to_string(nseg) + " nodes exceeds the 16384-node row limit");
+    hipLaunchKernelGGL(k_zero_rows, dim3(rows), dim3(BS), lds, s, tags, box, rowlen, kmax, off, len);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -1,5 +1,7 @@
 // Stencil form of the pressure-Poisson operator and the fused finest-level V-cycle
 // kernels (see stencil.hpp for the layout and the bit-identity argument).
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 #include "stencil.hpp"
@@ -387,10 +389,17 @@ static void tiles(const P5Dev &A, int &tx, int &nt) {
     nt = tx * ((A.ny + TY - 1) / TY);
 }
 
-void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s) {
+void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s,
+                   KernelProbe *probe) {
     int tx, nt;
     tiles(A, tx, nt);
-    hipLaunchKernelGGL(k_p5_pre, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, b, x2, r, tx, nt);
+    if (probe && probe->enabled()) {
+        hipEvent_t e0, e1;
+        probe->pair(e0, e1);
+        hipExtLaunchKernelGGL(k_p5_pre, dim3(xcd_grid(nt)), dim3(256), 0, s, e0, e1, 0, A, omega, b, x2, r, tx, nt);
+    } else {
+        hipLaunchKernelGGL(k_p5_pre, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, b, x2, r, tx, nt);
+    }
     CUIBM_LAUNCH_CHECK();
 }
 

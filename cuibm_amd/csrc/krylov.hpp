@@ -130,6 +130,9 @@ struct AmgParams {
     int max_levels = 25;
     double omega = 0.8;
     int npre = 2, npost = 2;
+    // restrict the finest level with P^T D^-1 instead of P^T: the hierarchy of the
+    // row-scaled D^-1 A, for matrices whose rows mix scales
+    bool scale0 = false;
 };
 
 // HIP events around the launches of one kernel inside the live solve: mark() before
@@ -152,6 +155,17 @@ class KernelProbe {
             ev_.push_back(e);
         }
         CUIBM_HIP(hipEventRecord(ev_[used_++], s));
+    }
+    // a start/stop pair for one launch made with hipExtLaunchKernel, which stamps them
+    // with the kernel's own begin and end (no dispatch latency inside the pair)
+    void pair(hipEvent_t &start, hipEvent_t &stop) {
+        while (ev_.size() < used_ + 2) {
+            hipEvent_t e;
+            CUIBM_HIP(hipEventCreate(&e));
+            ev_.push_back(e);
+        }
+        start = ev_[used_++];
+        stop = ev_[used_++];
     }
     int launches() const { return (int)(used_ / 2); }
     double total_ms() const {
@@ -224,7 +238,7 @@ class AmgPrecond : public Preconditioner {
 struct HostAmgLevel {
     HCsr A, P, R;
 };
-std::vector<HostAmgLevel> amg_build(const HCsr &A, int coarse_max, int max_levels);
+std::vector<HostAmgLevel> amg_build(const HCsr &A, int coarse_max, int max_levels, bool scale0 = false);
 std::vector<double> dense_inverse(const HCsr &A);
 
 // ---- BiCGSTAB (krylov.hip)

@@ -152,13 +152,16 @@ static HCsr download_csr(const DCsr<double> &A, hipStream_t s) {
 }
 
 static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p, const DCsr<double> &A, hipStream_t s,
-                                                    const P5Dev *fine = nullptr) {
+                                                    const P5Dev *fine = nullptr, bool scale0 = false,
+                                                    int coarse_max = 2500) {
     switch (p.preconditioner) {
         case NONE: return std::unique_ptr<Preconditioner>(new IdentityPrecond(A.n));
         case DIAGONAL: return std::unique_ptr<Preconditioner>(new JacobiPrecond(A, s));
         case SMOOTHED_AGGREGATION: {
             AmgParams ap;
             ap.precision = p.amgPrecision;
+            ap.scale0 = scale0;
+            ap.coarse_max = coarse_max;
             return std::unique_ptr<Preconditioner>(new AmgPrecond(download_csr(A, s), ap, s, fine));
         }
         default: throw Error("preconditioner AINV is not supported", ERR_UNSUPPORTED);
@@ -177,7 +180,8 @@ void NavierStokesSolver::buildPreconditioners() {
         op2.reset(new CsrOp(LHS2));
     }
     PC1 = make_precond(db->velocitySolve, LHS1, stream_);
-    PC2 = make_precond(db->PoissonSolve, LHS2, stream_, stencilLHS2() ? &fine : nullptr);
+    PC2 = make_precond(db->PoissonSolve, LHS2, stream_, stencilLHS2() ? &fine : nullptr, mixedScaleLHS2(),
+                       amgCoarseMax());
     ks1.reset(new Bicgstab(numUV));
     ks2.reset(new Bicgstab(numP));
     CUIBM_HIP(hipStreamSynchronize(stream_));
@@ -1175,7 +1179,10 @@ void luoIter::_pressure() {
 // luo_base.cu:137-147 with the matrices and preconditioners of the moved body
 void luoIter::updateSolver() {
     luo_base::updateSolver();
-    if (moving) initialiseLHS();
+    if (moving) {  // the reference rebuilds them in its setup phases; timed with the update here
+        ScopedTimer t(logger, "Update Solver");
+        initialiseLHS();
+    }
 }
 
 int luoIter::phase(const std::string &n) {

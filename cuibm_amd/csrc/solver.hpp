@@ -109,6 +109,12 @@ class NavierStokesSolver {
         return FadlunLhsDev{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     }
     virtual bool centrePin() const { return true; }
+    // LHS2 rows of different scales (unit-diagonal interpolation rows beside Poisson rows):
+    // the AMG hierarchy then restricts the finest level with P^T D^-1
+    virtual bool mixedScaleLHS2() const { return false; }
+    // largest coarsest AMG level of the Poisson hierarchy (solved with a dense inverse,
+    // whose set-up costs 2 n^3)
+    virtual int amgCoarseMax() const { return 2500; }
     DArray<double> *dfield(const std::string &name);
     const DArray<double> *dfield(const std::string &name) const;
     virtual DArray<int> *ifield(const std::string &name) { return nullptr; }
@@ -215,6 +221,10 @@ class luoIter : public luo_base {
   protected:
     bool centrePin() const override { return false; }
     bool stencilLHS2() const override { return false; }  // interpolation rows
+    bool mixedScaleLHS2() const override { return true; }
+    // a moving body rebuilds the hierarchy every step: a small coarsest level keeps the
+    // dense inverse cheap (2 * 300^3 flops against up to 2 * 2500^3)
+    int amgCoarseMax() const override { return moving ? 300 : 2500; }
     // row maps (-1 plain, t hybrid slot, -(t + 2) ghost slot), row buffers, row scales
     DArray<int> map1, map2, cnt, rowc, stats;
     DArray<double> rowv, ns1, ns2;

@@ -44,7 +44,9 @@ class PoissonStencil : public LinOp {
 // finest-level V-cycle pieces (omega-weighted Jacobi, D = diag(A)):
 //   pre:    x1 = omega D^-1 b, x2 = x1 + omega D^-1 (b - A x1), r = b - A x2
 //   sweep2: two sweeps x <- x + omega D^-1 (b - A x) (after the prolongation)
-void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s);
+// with an enabled probe, the launch carries a start/stop event pair of the probe
+void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s,
+                   KernelProbe *probe = nullptr);
 void launch_p5_sweep2(const P5Dev &A, double omega, const double *x, const double *b, double *xo, hipStream_t s);
 // one sweep xn = x + omega D^-1 (b - A x) (timing of the finest-level smoother)
 void launch_p5_sweep(const P5Dev &A, double omega, const double *x, const double *b, double *xn, hipStream_t s);

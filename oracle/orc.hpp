@@ -61,6 +61,9 @@ struct Amg {
     std::vector<double> coarse_inv;  // dense row-major inverse of the coarsest A
     double omega = 0.8;
     int npre = 2, npost = 2;
+    // restrict the finest level with P^T D^-1 (the hierarchy of the row-scaled D^-1 A);
+    // set for matrices whose rows mix scales (the embedded method's Poisson matrix)
+    bool scale0 = false;
     // coarse_max <= 0: amg_default_coarse_max (2500; the tests lower it to reach
     // several levels on small matrices)
     void setup(const Csr &A, int coarse_max = 0, int max_levels = 25);
@@ -73,7 +76,7 @@ struct Precond {
     int type = P_NONE;
     std::vector<double> dinv;
     Amg amg;
-    void build(const Csr &A, int type);
+    void build(const Csr &A, int type, bool amg_scale0 = false, int amg_coarse_max = 0);
     void apply(const double *r, double *z);
 };
 

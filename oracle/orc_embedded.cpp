@@ -393,7 +393,9 @@ void Solver::embedded_pass() {
     gn_pressure(false);
     hn_pressure();
     buildLHS2_embedded();
-    PC2.build(LHS2, P.precond2);
+    // interpolation rows (unit diagonal) beside Poisson rows; a moving body rebuilds the
+    // hierarchy every step on the product side, which then stops coarsening at 300 unknowns
+    PC2.build(LHS2, P.precond2, true, B.moving ? 300 : 0);
     update_rhs2_embedded();
     solve2();
     // luo_base/projectVelocity.cu:11-53

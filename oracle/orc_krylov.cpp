@@ -230,6 +230,8 @@ void Amg::setup(const Csr &A0, int coarse_max, int max_levels) {
             P.rp[i + 1] = (int)P.ci.size();
         }
         L.R = transpose(P, nagg);
+        if (scale0 && lv.empty())  // R = P^T D^-1: column j of P^T scaled by 1/a_jj
+            for (size_t k = 0; k < L.R.v.size(); k++) L.R.v[k] = L.R.v[k] * L.dinv[L.R.ci[k]];
         Csr AP = spgemm(A, P, nagg);
         Csr Ac = spgemm(L.R, AP, nagg);
         L.A = A;
@@ -282,10 +284,13 @@ static void vcycle(Amg &G, int l, const double *b, double *x) {
 
 void Amg::apply(const double *b, double *x) { vcycle(*this, 0, b, x); }
 
-void Precond::build(const Csr &A, int t) {
+void Precond::build(const Csr &A, int t, bool amg_scale0, int amg_coarse_max) {
     type = t;
     if (t == P_DIAGONAL) dinv = diag_inv(A);
-    else if (t == P_SMOOTHED_AGGREGATION) amg.setup(A);
+    else if (t == P_SMOOTHED_AGGREGATION) {
+        amg.scale0 = amg_scale0;
+        amg.setup(A, amg_coarse_max);
+    }
 }
 
 void Precond::apply(const double *r, double *z) {

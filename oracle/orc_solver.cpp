@@ -519,6 +519,10 @@ void orc_info(void *h, double *out) {
 // standalone linear-algebra entry points used by the solver tests
 // coarsest size of the AMG hierarchies set up from now on (<= 0 restores 2500)
 void orc_set_amg_coarse_max(int n) { orc::amg_default_coarse_max = n > 0 ? n : 2500; }
+// hierarchies built by orc_bicgstab and the inspection entry points below restrict the
+// finest level with P^T D^-1 while this is set (the embedded method's hierarchy)
+static bool g_scale0 = false;
+void orc_amg_scale0(int on) { g_scale0 = on != 0; }
 
 void *orc_csr(int n, const int *rp, const int *ci, const double *v) {
     auto *A = new orc::Csr();
@@ -533,7 +537,7 @@ void orc_csr_free(void *A) { delete (orc::Csr *)A; }
 int orc_bicgstab(void *A, double *x, const double *b, int precond, double rtol, int maxit, double *resid) {
     auto *M = (orc::Csr *)A;
     orc::Precond P;
-    P.build(*M, precond);
+    P.build(*M, precond, g_scale0);
     orc::SolveStat s = orc::bicgstab(*M, x, b, P, rtol, maxit);
     if (resid) *resid = s.resid;
     return s.converged ? s.iters : -1 - s.iters;
@@ -542,6 +546,7 @@ int orc_bicgstab(void *A, double *x, const double *b, int precond, double rtol, 
 // AMG hierarchy inspection: returns the number of levels; sizes[l] = n, nnzs[l] = nnz(A_l)
 int orc_amg_levels(void *A, int *sizes, int *nnzs, int cap) {
     orc::Amg G;
+    G.scale0 = g_scale0;
     G.setup(*(orc::Csr *)A);
     const int L = (int)G.lv.size();
     for (int l = 0; l < L && l < cap; l++) {
@@ -554,6 +559,7 @@ int orc_amg_levels(void *A, int *sizes, int *nnzs, int cap) {
 // one V-cycle x = M^-1 b (zero start)
 int orc_amg_vcycle(void *A, const double *b, double *x) {
     orc::Amg G;
+    G.scale0 = g_scale0;
     G.setup(*(orc::Csr *)A);
     G.apply(b, x);
     return (int)G.lv.size();
@@ -562,6 +568,7 @@ int orc_amg_vcycle(void *A, const double *b, double *x) {
 // Galerkin product of the first level: writes A_1 = R A P as dense (n1 x n1); returns n1
 int orc_amg_coarse_dense(void *A, int level, double *out, int cap) {
     orc::Amg G;
+    G.scale0 = g_scale0;
     G.setup(*(orc::Csr *)A);
     if (level < 0) level = (int)G.lv.size() - 1;
     if (level >= (int)G.lv.size()) return -1;
@@ -572,9 +579,25 @@ int orc_amg_coarse_dense(void *A, int level, double *out, int cap) {
         for (int k = Ac.rp[i]; k < Ac.rp[i + 1]; k++) out[(size_t)i * Ac.n + Ac.ci[k]] += Ac.v[k];
     return Ac.n;
 }
+// level `level` matrix as CSR (which = 0: A_l, 1: P_l, 2: R_l): returns the row count and
+// writes the entry count to *nnz; the arrays are filled only when they hold cap entries
+int orc_amg_level_csr(void *A, int level, int which, int *rp, int *ci, double *v, int cap, int *nnz) {
+    orc::Amg G;
+    G.scale0 = g_scale0;
+    G.setup(*(orc::Csr *)A);
+    if (level < 0 || level >= (int)G.lv.size()) return -1;
+    const orc::Csr &M = which == 0 ? G.lv[level].A : which == 1 ? G.lv[level].P : G.lv[level].R;
+    *nnz = M.nnz();
+    if (M.nnz() > cap || M.n + 1 > cap) return M.n;
+    std::copy(M.rp.begin(), M.rp.end(), rp);
+    std::copy(M.ci.begin(), M.ci.end(), ci);
+    std::copy(M.v.begin(), M.v.end(), v);
+    return M.n;
+}
 // dense P of the first level (n0 x n1)
 int orc_amg_prolongator_dense(void *A, double *out, int cap) {
     orc::Amg G;
+    G.scale0 = g_scale0;
     G.setup(*(orc::Csr *)A);
     if (G.lv.size() < 2) return -1;
     const orc::Csr &P = G.lv[0].P;

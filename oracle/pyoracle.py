@@ -79,6 +79,11 @@ def lib():
         L.orc_amg_vcycle.argtypes = [C.c_void_p, _DP, _DP]
         L.orc_set_amg_coarse_max.argtypes = [C.c_int]
         L.orc_set_amg_coarse_max.restype = None
+        L.orc_amg_scale0.restype = None
+        L.orc_amg_scale0.argtypes = [C.c_int]
+        L.orc_amg_level_csr.restype = C.c_int
+        L.orc_amg_level_csr.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP,
+                                        C.c_int, C.POINTER(C.c_int)]
         L.orc_amg_coarse_dense.restype = C.c_int
         L.orc_amg_coarse_dense.argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
         L.orc_amg_prolongator_dense.restype = C.c_int
@@ -178,6 +183,12 @@ def read_case(folder, overrides=None):
         else:
             sim[k] = v
     return case
+
+
+def set_amg_scale0(on):
+    """hierarchies of OrcCsr (inspection and bicgstab) restrict the finest level with
+    P^T D^-1 while on (the embedded method's hierarchy)"""
+    lib().orc_amg_scale0(1 if on else 0)
 
 
 def set_amg_coarse_max(n):
@@ -313,6 +324,21 @@ class OrcCsr:
         x = np.zeros(self.n)
         lib().orc_amg_vcycle(self.h, dptr(b), dptr(x))
         return x
+
+    def level_csr(self, level, which=0, cap=50_000_000):
+        """hierarchy matrix of level `level` as scipy CSR (which: 0 A, 1 P, 2 R)"""
+        import scipy.sparse as sp
+        IP = C.POINTER(C.c_int)
+        rp = np.zeros(cap, dtype=np.int32)
+        ci = np.zeros(cap, dtype=np.int32)
+        v = np.zeros(cap)
+        nnz = C.c_int(0)
+        n = lib().orc_amg_level_csr(self.h, level, which, rp.ctypes.data_as(IP), ci.ctypes.data_as(IP), dptr(v), cap,
+                                    C.byref(nnz))
+        if n < 0 or nnz.value > cap:
+            raise ValueError(f"level {level} unavailable (n={n}, nnz={nnz.value})")
+        ncols = int(ci[: nnz.value].max()) + 1 if nnz.value else 0
+        return sp.csr_matrix((v[: nnz.value], ci[: nnz.value], rp[: n + 1]), shape=(n, max(ncols, n if which == 0 else ncols)))
 
     def coarse_dense(self, level=1, cap=4_000_000):
         """dense Galerkin operator of hierarchy level `level` (-1: coarsest)"""

@@ -65,7 +65,8 @@ def test_embedded_matrices_match_oracle(name):
     _no_anomalies(g, o)
 
 
-@pytest.mark.parametrize("name,steps", [("cylinderRe40_small_embedded", 10), ("cylinderRe40_embedded", 5)])
+@pytest.mark.parametrize("name,steps", [("cylinderRe40_small_embedded", 10), ("cylinderRe40_embedded", 5),
+                                        ("cylinderRe40_462_embedded", 2)])
 def test_embedded_steps_match_oracle(name, steps):
     g, o = pair(name, {"velocityTol": 1e-12, "poissonTol": 1e-12})
     for k in range(steps):

@@ -598,3 +598,38 @@ def test_embedded_plain_rows_are_the_navier_stokes_rows_over_their_diagonal():
     far = np.nonzero((gp == -1) & (hp == -1))[0]
     d = (E2[far] - P2[far])
     assert abs(d).max() == 0
+
+
+def test_embedded_poisson_hierarchy_restricts_with_scaled_transpose():
+    """The embedded Poisson matrix mixes unit-diagonal interpolation rows with Poisson rows
+    of diagonal 4 dt/h^2 (40 on the reference's 462x462 validation/error/cylinder/
+    embedded4 case). With R = P^T its Galerkin level 1 has negative diagonals and the
+    V-cycle amplifies; the embedded hierarchy restricts its finest level with P^T D^-1.
+    Pins: A_1 equals P^T D^-1 A P formed here from the level's own P (a dropped or
+    inverted D fails), every coarse diagonal is positive, and BiCGSTAB with it converges
+    where R = P^T does not."""
+    c = po.read_case(case("cylinderRe40_462_embedded"))
+    o = po.Oracle(c)
+    o.step(1)
+    assert o.info()["iter2"] < 200
+    A = _emb_csr(o, 2)
+    d = A.diagonal()
+    assert (np.abs(d - 1.0) < 1e-12).sum() > 100 and d.max() > 30  # rows of both scales
+    try:
+        po.set_amg_scale0(False)
+        M0 = po.OrcCsr(A)
+        assert (M0.level_csr(1).diagonal() <= 0).any()
+        po.set_amg_scale0(True)
+        M = po.OrcCsr(A)
+        P = M.level_csr(0, 1)
+        A1 = M.level_csr(1)
+        ref = (P.T @ sp.diags(1.0 / d) @ A @ P).tocsr()
+        diff = (A1 - ref).tocsr()
+        assert np.abs(diff.data).max(initial=0.0) <= 1e-12 * np.abs(ref.data).max()
+        for lev in range(1, len(M.amg_levels()[0])):
+            assert (M.level_csr(lev).diagonal() > 0).all(), lev
+        b = A @ np.random.default_rng(3).standard_normal(A.shape[0])
+        x, it, r = M.bicgstab(b, precond="SMOOTHED_AGGREGATION", rtol=1e-6, maxit=300)
+        assert 0 <= it < 100 and np.linalg.norm(b - A @ x) <= 1e-6 * np.linalg.norm(b) * 1.0001
+    finally:
+        po.set_amg_scale0(False)
