@@ -10,7 +10,7 @@ namespace cuibm {
 
 namespace {
 
-constexpr int TX = 64, TY = 16;  // cells of a tile (one wavefront wide)
+constexpr int TX = 64, TY = 8;  // cells of a tile (one wavefront wide; 64x8 measured best of 32-128 x 4-32)
 constexpr int W1 = TX + 2, H1 = TY + 2, W2 = TX + 4, H2 = TY + 4;
 
 // blocks arrive at the 8 XCDs round-robin; tile t = xcd * per + k gives each XCD a
