@@ -177,6 +177,46 @@ def published_runs(cu):
     return rows
 
 
+def ensemble_bench(cu, n, steps, workers, check):
+    """n seeded cylinder cases of 96^2..512^2 cells and Re 40..200 (cuibm_amd.ensemble),
+    run concurrently with the FP32-valued hierarchy; every (n // check)-th case is then
+    run alone with the FP64 hierarchy and its drag coefficient compared"""
+    import tempfile
+    from cuibm_amd import ensemble as ens
+
+    folder = tempfile.mkdtemp(prefix="cuibm_ensemble_")
+    paths, cases = ens.write_cases(folder, n=n, steps=steps)
+    cells = sum(c[0] ** 2 for c in cases)
+    log(f"[bench] ensemble: {n} cases, {cells} cells, {steps} steps each, {workers} host workers")
+    r = cu.ensemble_run(paths, "-amgPrecision", "32", steps=steps, workers=workers)
+    rate = cells * steps / r["steps_wall_s"]
+    log(f"[bench] ensemble: {rate / 1e6:.1f} M cell-updates/s over the steps ({r['steps_wall_s']:.2f} s), "
+        f"{r['wall_s']:.2f} s with set-up")
+    picked = list(range(0, n, max(1, n // check)))[:check]
+    dcd, t_alone = [], 0.0
+    for i in picked:
+        s = cu.Solver(paths[i], "-amgPrecision", "64", diagnostics=False)
+        s.initialise()
+        ms = s.time_steps(steps)
+        t_alone += ms * 1e-3
+        dcd.append(abs(2.0 * s.info()["forceX"] - r["cd"][i]))
+        s.close()
+        log(f"[bench]   case {i} ({cases[i][0]}^2, Re {cases[i][1]:.1f}) alone in FP64: |dC_D| {dcd[-1]:.2e}")
+    alone_rate = sum(cases[i][0] ** 2 for i in picked) * steps / t_alone
+    return {
+        "metric": "aggregate cell-updates/s, ensemble of independent cylinder cases on one GPU",
+        "value": round(rate, 1), "unit": "cell-updates/s", "higher_is_better": True, "n_gpus": 1,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n} impulsively started cylinders, edge 96..512 cells (seeded), Re 40..200, "
+                               f"h=D/40 on [-0.6,0.6] stretched to [-10,10], dt=0.01, Luo external, Poisson FP64 "
+                               f"BiCGSTAB rtol 1e-8 + SA-AMG with FP32 values, velocity BiCGSTAB+Jacobi rtol 1e-8",
+                   "cases": n, "steps": steps, "cells": cells, "workers": workers},
+        "steps_wall_s": round(r["steps_wall_s"], 3), "wall_s_with_setup": round(r["wall_s"], 3),
+        "alone_fp64_cell_updates_per_s": round(alone_rate, 1),
+        "check": {"cases": picked, "max_abs_dcd_vs_alone_fp64": max(dcd)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +235,12 @@ def main():
     ap.add_argument("--published", action="store_true",
                     help="run the reference's published whole-run workloads in full and print their times "
                          "beside the published ones (one JSON line; not the headline metric)")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="run N seeded cylinder cases concurrently and print the aggregate cell-update rate "
+                         "(one JSON line; not the headline)")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-workers", type=int, default=16)
+    ap.add_argument("--ensemble-check", type=int, default=8, help="cases re-run alone in FP64 for the C_D check")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -206,6 +252,10 @@ def main():
     import cuibm_amd as cu
 
     case = os.path.join(cu.CASES, CASE)
+    if args.ensemble > 0:
+        print(json.dumps(ensemble_bench(cu, args.ensemble, args.ensemble_steps, args.ensemble_workers,
+                                        args.ensemble_check)), flush=True)
+        return
     if args.published:
         rows = published_runs(cu)
         print(json.dumps({"metric": "seconds per published whole run (set-up + all steps, no file output)",

@@ -101,6 +101,9 @@ def lib():
     L.cuibm_bicgstab.argtypes = [C.c_void_p, _DP, _DP, C.c_int, C.c_int, C.c_double, C.c_int, _IP, _DP]
     L.cuibm_amg_levels.argtypes = [C.c_void_p, _IP, _IP, C.c_int]
     L.cuibm_amg_vcycle.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
+    L.cuibm_ensemble_run.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int,
+                                     C.POINTER(C.c_longlong), _DP, _DP, _DP, _DP, C.c_char_p, C.c_int]
+    L.cuibm_ensemble_run.restype = C.c_int
     _lib = L
     return L
 
@@ -118,6 +121,25 @@ def device_synchronize():
 
 def _dp(a):
     return a.ctypes.data_as(_DP)
+
+
+def ensemble_run(case_folders, *args, steps=0, workers=8):
+    """Run independent cases to completion concurrently (cuibm_ensemble_run): each on its
+    own solver and HIP stream, driven by `workers` host threads; `args` are common
+    command-line overrides. Returns a dict with per-case cells, cd, cl and the wall times."""
+    n = len(case_folders)
+    folders = (C.c_char_p * n)(*[os.fsencode(str(f)) for f in case_folders])
+    argv = (C.c_char_p * max(len(args), 1))(*[os.fsencode(str(a)) for a in args])
+    cells = np.zeros(n, dtype=np.int64)
+    cd, cl = np.zeros(n), np.zeros(n)
+    wall, wall_steps = C.c_double(0), C.c_double(0)
+    err = C.create_string_buffer(1024)
+    rc = lib().cuibm_ensemble_run(n, folders, len(args), argv, int(steps), int(workers),
+                                  cells.ctypes.data_as(C.POINTER(C.c_longlong)), _dp(cd), _dp(cl), C.byref(wall),
+                                  C.byref(wall_steps), err, len(err))
+    if rc < 0:
+        raise CuibmError(rc, err.value.decode(errors="replace"))
+    return {"cells": cells, "cd": cd, "cl": cl, "wall_s": wall.value, "steps_wall_s": wall_steps.value}
 
 
 class Solver:

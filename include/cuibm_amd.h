@@ -167,6 +167,22 @@ int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms,
  * the timing harness brackets its timed region with it. No reference counterpart. */
 int cuibm_device_synchronize(void);
 
+/* ---- ensembles of independent cases ------------------------------------------------ */
+
+/* No counterpart in the reference, which runs one case per process (src/cuIBM.cu:30-82):
+ * run n cases to completion concurrently on the current device. Each case is created from
+ * case_folders[i] with the common overrides argc/argv (as cuibm_create), initialised and
+ * stepped `steps` times (steps <= 0: its own nt) on its own solver and HIP stream.
+ * `workers` host threads (1..64) drive the cases; each case goes to the least loaded
+ * thread, largest cells x steps first. Outputs (arrays of n, may be null): cells[i] =
+ * nx*ny, cd[i] = 2 Fx and cl[i] = 2 Fy after the last step. *wall_s = seconds from the
+ * first create to the last step, *steps_wall_s = seconds from the moment every case is
+ * initialised to the last step. On failure returns the error code of the lowest-numbered
+ * failing case and writes its message to err (err_len bytes, may be null). */
+int cuibm_ensemble_run(int n, const char *const *case_folders, int argc, const char *const *argv, int steps,
+                       int workers, long long *cells, double *cd, double *cl, double *wall_s, double *steps_wall_s,
+                       char *err, int err_len);
+
 /* ---- linear algebra (replaces cusp::krylov::bicgstab / cusp::precond) ------------- */
 
 /* upload a CSR matrix (n x n, 0-based, sorted columns) */
