@@ -15,6 +15,7 @@
 // Neumann Poisson operator this lets BiCGSTAB iteration counts grow by orders of
 // magnitude over a run and end in breakdown.
 #include <algorithm>
+#include <cstdint>
 
 #include "krylov.hpp"
 #include "stencil.hpp"
@@ -147,31 +148,50 @@ void launch_csr_op(int lpr, const DCsr<V> &A, const double *x, const double *b, 
     CUIBM_LAUNCH_CHECK();
 }
 
-// x = Ainv b on the coarsest level: b staged in LDS (n doubles of dynamic LDS), one
-// wavefront per row with lanes across the row-major inverse (coalesced), four
-// independent partial sums per lane combined in a fixed order, then a fixed shuffle
-// tree across the wavefront
+// x = Ainv b on the coarsest level. Ainv is row-major with a row stride ld that is a
+// multiple of 4 (zero padding), so each lane loads four consecutive matrix values at a
+// time (one 16-byte load for FP32, two for FP64) and the matching four values of b, which
+// every wavefront reads from cache; one wavefront per row, four partial sums per lane
+// combined in a fixed order, then a fixed shuffle tree across the wavefront
 template <class V>
-__global__ void __launch_bounds__(256) k_dense(int n, const V *__restrict__ Ainv, const double *__restrict__ b,
+__device__ __forceinline__ void load4(const V *p, double (&v)[4]);
+template <>
+__device__ __forceinline__ void load4<float>(const float *p, double (&v)[4]) {
+    const float4 q = *reinterpret_cast<const float4 *>(p);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+template <>
+__device__ __forceinline__ void load4<double>(const double *p, double (&v)[4]) {
+    const double2 q0 = reinterpret_cast<const double2 *>(p)[0], q1 = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+}
+
+template <class V>
+__global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restrict__ Ainv, const double *__restrict__ b,
                                                double *__restrict__ x, const double *__restrict__ stop) {
-    const double stopv = stop ? *stop : 0.0;  // read at once, tested before the barrier
-    extern __shared__ double sb[];
-    for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = b[j];
-    if (stopv != 0.0) return;  // uniform across the block
-    __syncthreads();
+    const double stopv = stop ? *stop : 0.0;
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;  // uniform per wavefront, after the only barrier
-    const V *a = Ainv + (size_t)row * n;
+    if (row >= n) return;  // no barrier in this kernel
+    const V *a = Ainv + (size_t)row * ld;
+    const int n4 = n & ~3;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    int j = lane;
-    for (; j + 192 < n; j += 256) {
-        s0 += (double)a[j] * sb[j];
-        s1 += (double)a[j + 64] * sb[j + 64];
-        s2 += (double)a[j + 128] * sb[j + 128];
-        s3 += (double)a[j + 192] * sb[j + 192];
+    bool tested = false;
+    for (int j = 4 * lane; j < n4; j += 256) {
+        double av[4], bv[4];
+        load4<V>(a + j, av);
+        load4<double>(b + j, bv);
+        if (!tested) {
+            if (stopv != 0.0) return;
+            tested = true;
+        }
+        s0 += av[0] * bv[0];
+        s1 += av[1] * bv[1];
+        s2 += av[2] * bv[2];
+        s3 += av[3] * bv[3];
     }
-    for (; j < n; j += 64) s0 += (double)a[j] * sb[j];
+    if (stopv != 0.0) return;
+    if (lane < n - n4) s0 += (double)a[n4 + lane] * b[n4 + lane];
     double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
@@ -193,7 +213,7 @@ template <class V>
 struct AmgDev {
     std::vector<AmgLevelDev<V>> lv;
     DArray<V> coarse_inv;
-    int nc = 0;
+    int nc = 0, ld = 0;
     AmgParams prm;
     bool use_fine = false;  // finest level in stencil form (fused kernels)
     P5Dev fine{};
@@ -234,7 +254,10 @@ struct AmgDev {
         nc = h.back().A.n;
         if (nc > 6144) throw Error("AMG coarsening stalled at " + std::to_string(nc) + " unknowns", ERR_GENERIC);
         const std::vector<double> inv = dense_inverse(h.back().A);
-        std::vector<V> invt(inv.begin(), inv.end());
+        ld = (nc + 3) & ~3;  // row stride, zero-padded to a multiple of 4 values
+        std::vector<V> invt((size_t)nc * ld, V(0));
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nc; j++) invt[(size_t)i * ld + j] = (V)inv[(size_t)i * nc + j];
         coarse_inv.upload(invt, s);
         CUIBM_HIP(hipStreamSynchronize(s));
     }
@@ -245,7 +268,9 @@ struct AmgDev {
         const int n = L.n;
         const double w = prm.omega;
         if (l + 1 == lv.size()) {
-            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), n * sizeof(double), s, n, coarse_inv.p(), b, x,
+            if ((reinterpret_cast<uintptr_t>(b) & 15) != 0)  // the kernel reads b four values at a time
+                throw Error("coarsest AMG right-hand side is not 16-byte aligned", ERR_ARGUMENT);
+            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), 0, s, n, ld, coarse_inv.p(), b, x,
                                stop);
             CUIBM_LAUNCH_CHECK();
             return;

