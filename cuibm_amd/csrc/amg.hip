@@ -75,6 +75,49 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
     }
 }
 
+// Restricted residual y = R (b - A x) as one product with G = [R | -R A]: column c < nf
+// of G multiplies b[c], column nf + j multiplies x[j]. Replaces the residual and the
+// restriction launches; the residual vector itself is never formed.
+template <class V, int LPR>
+__global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restrict__ rp, const int *__restrict__ ci,
+                                             const V *__restrict__ v, const double *__restrict__ x,
+                                             const double *__restrict__ b, double *__restrict__ y,
+                                             const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
+    const int lane = threadIdx.x & (LPR - 1);
+    const int per_block = BS / LPR;
+    for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+        double s = 0;
+        const int e = rp[i + 1];
+        for (int k = rp[i] + lane; k < e; k += LPR) {
+            const int c = ci[k];
+            s += (double)v[k] * (c < nf ? b[c] : x[c - nf]);
+        }
+        if (LPR > 1)
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
+        if (stopv != 0.0) return;
+        if (lane == 0) y[i] = s;
+    }
+}
+
+template <class V>
+void launch_rres(int lpr, const DCsr<V> &G, int nf, const double *x, const double *b, double *y, hipStream_t s,
+                 const double *stop) {
+    const int n = G.n;
+    if (n <= 0) return;
+    const long blocks = ((long)n * lpr + BS - 1) / BS;
+    const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
+    switch (lpr) {
+        case 1: hipLaunchKernelGGL((k_rres<V, 1>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 4: hipLaunchKernelGGL((k_rres<V, 4>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 8: hipLaunchKernelGGL((k_rres<V, 8>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 16: hipLaunchKernelGGL((k_rres<V, 16>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        default: hipLaunchKernelGGL((k_rres<V, 32>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+    }
+    CUIBM_LAUNCH_CHECK();
+}
+
 // Both pre-smoothing sweeps of a CSR level in one pass: x2 = x1 + omega D^-1 (b - A x1)
 // with x1 = omega D^-1 b evaluated at the row's neighbours on the fly. (Fusing the
 // prolongation into the first post-sweep the same way was measured slower: every row
@@ -203,8 +246,8 @@ __global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restric
 template <class V>
 struct AmgLevelDev {
     int n = 0;
-    int lanesA = 1, lanesP = 1, lanesR = 1;  // lanes per row of each operator
-    DCsr<V> A, P, R;
+    int lanesA = 1, lanesP = 1, lanesR = 1, lanesG = 1;  // lanes per row of each operator
+    DCsr<V> A, P, R, G;
     DArray<V> dinv;
     DArray<double> b, x, xt, r;
 };
@@ -233,6 +276,10 @@ struct AmgDev {
                 upload_csr(h[l].R, L.R, s);
                 L.lanesP = lanes_for(h[l].P);
                 L.lanesR = lanes_for(h[l].R);
+                if (!stencil_level) {
+                    upload_csr(h[l].G, L.G, s);
+                    L.lanesG = lanes_for(h[l].G);
+                }
             }
             if (!stencil_level) {
                 std::vector<V> d(L.n, V(0));
@@ -291,8 +338,7 @@ struct AmgDev {
             // post-sweeps xt -> r -> x
             AmgLevelDev<V> &C = lv[l + 1];
             launch_pre2<V>(L.lanesA, L.A, L.dinv.p(), b, L.xt.p(), w, s, stop);
-            launch_csr_op<V, OP_RESID>(L.lanesA, L.A, L.xt.p(), b, nullptr, L.r.p(), 0.0, s, stop);
-            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
+            launch_rres<V>(L.lanesG, L.G, n, L.xt.p(), b, C.b.p(), s, stop);  // R (b - A x2)
             vcycle(l + 1, C.b.p(), C.x.p(), s);
             launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s, stop);
             launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), b, L.dinv.p(), L.r.p(), w, s, stop);
@@ -357,6 +403,7 @@ AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P
         L.A.v.clear(); L.A.v.shrink_to_fit();
         L.P.v.clear(); L.P.v.shrink_to_fit();
         L.R.v.clear(); L.R.v.shrink_to_fit();
+        L.G.v.clear(); L.G.v.shrink_to_fit();
     }
 }
 
@@ -447,6 +494,14 @@ double AmgPrecond::bytes_per_apply() const {
         const double nc = H[l + 1].A.n, nnzP = (double)H[l].P.ci.size();
         if (l == 0 && fine_) {
             bytes += level0_pre_bytes() + spmv(nc, n, nnzP) + level0_post_bytes();  // pre, restriction, post
+            continue;
+        }
+        if (prm_.npre == 2 && prm_.npost == 2) {  // the fused (2,2) sequence of vcycle()
+            const double nnzG = (double)H[l].G.ci.size();
+            bytes += spmv(n, n, nnz) + n * sv + n * 8;                    // both pre-sweeps (+ dinv, b)
+            bytes += nnzG * (sv + 4) + 4 * (nc + 1) + 2 * n * 8 + nc * 8; // restricted residual: G, b, x2
+            bytes += spmv(n, nc, nnzP) + n * 8;                           // prolong-add
+            bytes += 2 * (spmv(n, n, nnz) + n * sv + n * 8);              // post-sweeps (+ dinv, b)
             continue;
         }
         bytes += n * sv + 2 * n * 8;                                   // first sweep: dinv, b, x

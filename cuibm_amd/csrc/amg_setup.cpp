@@ -185,6 +185,24 @@ std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_leve
             for (size_t k = 0; k < L.R.v.size(); k++) L.R.v[k] = L.R.v[k] * (1.0 / d[L.R.ci[k]]);
         const HCsr AP = multiply(A, P);
         HCsr Ac = multiply(L.R, AP);
+        {  // G = [R | -R A]
+            const HCsr RA = multiply(L.R, A);
+            HCsr &G = L.G;
+            G.n = L.R.n;
+            G.ncols = 2 * A.n;
+            G.rp.assign(G.n + 1, 0);
+            for (int c = 0; c < G.n; c++) {
+                for (int k = L.R.rp[c]; k < L.R.rp[c + 1]; k++) {
+                    G.ci.push_back(L.R.ci[k]);
+                    G.v.push_back(L.R.v[k]);
+                }
+                for (int k = RA.rp[c]; k < RA.rp[c + 1]; k++) {
+                    G.ci.push_back(A.n + RA.ci[k]);
+                    G.v.push_back(-RA.v[k]);
+                }
+                G.rp[c + 1] = (int)G.ci.size();
+            }
+        }
         L.A = A;
         L.P = P;
         lv.push_back(std::move(L));

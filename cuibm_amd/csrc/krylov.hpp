@@ -237,6 +237,9 @@ class AmgPrecond : public Preconditioner {
 // host smoothed-aggregation set-up (amg_setup.cpp)
 struct HostAmgLevel {
     HCsr A, P, R;
+    // G = [R | -R A] (A.n + A.n columns): the restricted residual R (b - A x) as one
+    // product with the pair (b, x), on every level but the coarsest
+    HCsr G;
 };
 std::vector<HostAmgLevel> amg_build(const HCsr &A, int coarse_max, int max_levels, bool scale0 = false);
 std::vector<double> dense_inverse(const HCsr &A);
