@@ -384,6 +384,144 @@ void PoissonStencil::apply(const double *x, double *y, hipStream_t s) const {
     CUIBM_LAUNCH_CHECK();
 }
 
+// ---- velocity operator (LHS1) in stencil form ---------------------------------------
+
+// the coupling magnitudes and diagonal terms of LHS1.cu:24-351, one thread per I or J:
+// u block: E/W from dx[I], dx[I+1] for I < nx-1, N/S from dy[J], dy[J+-1] with the
+// doubled-width ghost term at J = ny-1 (N) and J = 0 (S); v block: E/W from dx[I],
+// dx[I+-1] with the ghost term at I = nx-1 (E) and I = 0 (W), N/S from dy[J], dy[J+1]
+__global__ void k_v5_coef(int nx, int ny, const double *__restrict__ dx, const double *__restrict__ dy, double nu,
+                          double dt, double *__restrict__ c, int m) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double *uoE = c, *uoW = c + m, *udE = c + 2 * m, *udW = c + 3 * m;
+    double *uoN = c + 4 * m, *uoS = c + 5 * m, *udN = c + 6 * m, *udS = c + 7 * m;
+    double *voE = c + 8 * m, *voW = c + 9 * m, *vdE = c + 10 * m, *vdW = c + 11 * m;
+    double *voN = c + 12 * m, *voS = c + 13 * m, *vdN = c + 14 * m, *vdS = c + 15 * m;
+    if (k < nx - 1) {  // u, per I
+        const int I = k;
+        const double ce = 0.5 * dt * nu * (1 / (dx[I + 1] * (dx[I + 1] + dx[I]) * 0.5));
+        const double cw = 0.5 * dt * nu * (1 / (dx[I] * (dx[I + 1] + dx[I]) * 0.5));
+        uoE[I] = ce; udE[I] = ce;
+        uoW[I] = cw; udW[I] = cw;
+    }
+    if (k < ny) {  // u, per J
+        const int J = k;
+        const double nN = J != ny - 1 ? 0.5 * dt * nu * (1 / (dy[J] * (dy[J + 1] + dy[J]) * 0.5)) : 0.0;
+        const double nS = J != 0 ? 0.5 * dt * nu * (1 / (dy[J] * (dy[J - 1] + dy[J]) * 0.5)) : 0.0;
+        const double gh = 0.5 * dt * nu * (1 / (dy[J] * (dy[J] + dy[J]) * 0.5));
+        uoN[J] = nN; udN[J] = J != ny - 1 ? nN : gh;
+        uoS[J] = nS; udS[J] = J != 0 ? nS : gh;
+    }
+    if (k < nx) {  // v, per I
+        const int I = k;
+        const double nE = I != nx - 1 ? 0.5 * dt * nu * (1 / (dx[I] * (dx[I] + dx[I + 1]) * 0.5)) : 0.0;
+        const double nW = I != 0 ? 0.5 * dt * nu * (1 / (dx[I] * (dx[I] + dx[I - 1]) * 0.5)) : 0.0;
+        const double gh = 0.5 * dt * nu * (1 / (dx[I] * (dx[I] + dx[I]) * 0.5));
+        voE[I] = nE; vdE[I] = I != nx - 1 ? nE : gh;
+        voW[I] = nW; vdW[I] = I != 0 ? nW : gh;
+    }
+    if (k < ny - 1) {  // v, per J
+        const int J = k;
+        const double cn = 0.5 * dt * nu * (1 / (dy[J + 1] * (dy[J] + dy[J + 1]) * 0.5));
+        const double cs = 0.5 * dt * nu * (1 / (dy[J] * (dy[J] + dy[J + 1]) * 0.5));
+        voN[J] = cn; vdN[J] = cn;
+        voS[J] = cs; vdS[J] = cs;
+    }
+}
+
+// (LHS1 x)_i for the packed index i, in the CSR column order S, W, C, E, N
+__device__ __forceinline__ double v5_row(const V5Dev &A, int i, const double *__restrict__ x) {
+    const V5Block &B = i < A.v.off0 ? A.u : A.v;
+    const int k = i - B.off0, I = k % B.ni, J = k / B.ni;
+    const double d = (((1.0 + B.dE[I]) + B.dW[I]) + B.dN[J]) + B.dS[J];
+    double s = 0;
+    if (J != 0) s += (-B.oS[J]) * x[i - B.ni];
+    if (I != 0) s += (-B.oW[I]) * x[i - 1];
+    s += d * x[i];
+    if (I != B.ni - 1) s += (-B.oE[I]) * x[i + 1];
+    if (J != B.nj - 1) s += (-B.oN[J]) * x[i + B.ni];
+    return s;
+}
+
+// y = A x with block partials of (y . a) and (y . y): the loop, grid and reduction of
+// the CSR k_spmv_dot, so the partials are the CSR path's
+__global__ void __launch_bounds__(BS) k_v5_spmv_dot(V5Dev A, int n, const double *__restrict__ x,
+                                                    double *__restrict__ y, const double *__restrict__ a, int want_yy,
+                                                    double *__restrict__ part) {
+    const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
+    double acc[2] = {0, 0};
+    bool tested = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double yi = v5_row(A, i, x);
+        if (!tested) {
+            if (stopv != 0.0) return;  // every thread returns here or below: no barrier is reached
+            tested = true;
+        }
+        if (y) y[i] = yi;
+        acc[0] += yi * a[i];
+        acc[1] += yi * yi;
+    }
+    if (stopv != 0.0) return;
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        if (want_yy) part[RED_GRID + blockIdx.x] = acc[1];
+    }
+}
+
+// r = b - A x, p = rs = r, partials of (r . r) and (b . b): as the CSR k_bicg_init
+__global__ void __launch_bounds__(BS) k_v5_init(V5Dev A, int n, const double *__restrict__ x,
+                                                const double *__restrict__ b, double *__restrict__ r,
+                                                double *__restrict__ p, double *__restrict__ rs,
+                                                double *__restrict__ part) {
+    double acc[2] = {0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double bi = b[i];
+        const double ri = bi - v5_row(A, i, x);
+        r[i] = ri;
+        p[i] = ri;
+        rs[i] = ri;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        part[RED_GRID + blockIdx.x] = acc[1];
+    }
+}
+
+VelocityStencil::VelocityStencil(const GridDev &g, double nu, double dt, hipStream_t s)
+    : nx_(g.nx), ny_(g.ny), nu_((g.nx - 1) * g.ny), nv_(g.nx * (g.ny - 1)) {
+    const int m = std::max(nx_, ny_);
+    c_.alloc((size_t)16 * m);
+    hipLaunchKernelGGL(k_v5_coef, dim3((m + BS - 1) / BS), dim3(BS), 0, s, nx_, ny_, g.dx, g.dy, nu, dt, c_.p(), m);
+    CUIBM_LAUNCH_CHECK();
+}
+
+V5Dev VelocityStencil::dev() const {
+    const int m = std::max(nx_, ny_);
+    const double *c = c_.p();
+    V5Dev d;
+    d.u = V5Block{nx_ - 1, ny_, 0, c, c + m, c + 4 * m, c + 5 * m, c + 2 * m, c + 3 * m, c + 6 * m, c + 7 * m};
+    d.v = V5Block{nx_, ny_ - 1, nu_, c + 8 * m, c + 9 * m, c + 12 * m, c + 13 * m, c + 10 * m, c + 11 * m, c + 14 * m,
+                  c + 15 * m};
+    d.stop = stop_;
+    return d;
+}
+
+void VelocityStencil::spmv_dot(const double *x, double *y, const double *a, bool want_yy, double *part,
+                               hipStream_t s) const {
+    hipLaunchKernelGGL(k_v5_spmv_dot, dim3(RED_GRID), dim3(BS), 0, s, dev(), n(), x, y, a, (int)want_yy, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void VelocityStencil::init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
+                           hipStream_t s) const {
+    hipLaunchKernelGGL(k_v5_init, dim3(RED_GRID), dim3(BS), 0, s, dev(), n(), x, b, r, p, rs, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
 static void tiles(const P5Dev &A, int &tx, int &nt) {
     tx = (A.nx + TX - 1) / TX;
     nt = tx * ((A.ny + TY - 1) / TY);

@@ -170,7 +170,9 @@ static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p,
 
 // newPrecon.cu:17-35 / preconditioner.h:84-111
 void NavierStokesSolver::buildPreconditioners() {
-    op1.reset(new CsrOp(LHS1));
+    // LHS1 rows are the plain formula where LHS2's are (plain and Luo external): stencil form
+    if (stencilLHS2()) op1.reset(new VelocityStencil(grid(), nu, dt, stream_));
+    else op1.reset(new CsrOp(LHS1));
     P5Dev fine{};
     if (stencilLHS2()) {  // LHS2 rows are the plain formula: use the stencil form
         auto *p5 = new PoissonStencil(grid(), dt, centrePin(), stream_);
@@ -409,6 +411,10 @@ int NavierStokesSolver::phase(const std::string &n) {
     else if (n == "poisson_operator") {  // rhs2 = A p with the operator the Poisson solve uses
         DArray<double> part(2 * RED_GRID);
         op2->spmv_dot(pressure.p(), rhs2.p(), pressure.p(), false, part.p(), stream_);
+        CUIBM_HIP(hipStreamSynchronize(stream_));
+    } else if (n == "velocity_operator") {  // rhs1 = A uhat with the operator the velocity solve uses
+        DArray<double> part(2 * RED_GRID);
+        op1->spmv_dot(uhat.p(), rhs1.p(), uhat.p(), false, part.p(), stream_);
         CUIBM_HIP(hipStreamSynchronize(stream_));
     } else if (n == "rhs1_only") {  // the fused rhs1 sweep on the current N/Nold, without the swap
         launch_rhs1(u.p(), Nold.p(), N.p(), rhs1.p(), L.p(), bc1.p(), grid(), bcs(), nu, dt, fadlunRhs(), stream_);

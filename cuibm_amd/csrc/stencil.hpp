@@ -41,6 +41,40 @@ class PoissonStencil : public LinOp {
     DArray<double> cE_, cW_, cN_, cS_, dinv_;
 };
 
+// The velocity operator LHS1 = I - 0.5 dt nu Lap (NavierStokes/kernels/LHS1.cu:24-351) of
+// the plain and Luo-external solvers in the same form: it is block diagonal in (u, v), and
+// in each block the east/west couplings depend on I alone and the north/south ones on J
+// alone. Per block and direction, `off` holds the coupling magnitude (the entry is its
+// negative) and `dg` the term the row's diagonal adds for that direction, which differs
+// from `off` at a wall (the reference's ghost-cell diagonal). The diagonal is
+// ((1 + dgE[I]) + dgW[I]) + dgN[J]) + dgS[J], the assembly's order, so products equal
+// the CSR product with LHS1 bit for bit.
+struct V5Block {
+    int ni, nj, off0;                   // nodes per row and per column, first packed index
+    const double *oE, *oW, *oN, *oS;    // coupling magnitudes (o*[I] or o*[J])
+    const double *dE, *dW, *dN, *dS;    // diagonal terms
+};
+struct V5Dev {
+    V5Block u, v;
+    const double *stop = nullptr;  // the kernels return at once while *stop != 0
+};
+
+class VelocityStencil : public LinOp {
+  public:
+    VelocityStencil(const GridDev &g, double nu, double dt, hipStream_t s);
+    int n() const override { return nu_ + nv_; }
+    void spmv_dot(const double *x, double *y, const double *a, bool want_yy, double *part,
+                  hipStream_t s) const override;
+    void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
+              hipStream_t s) const override;
+    double spmv_bytes() const override { return 16.0 * n() + 64.0 * (nx_ + ny_); }
+    V5Dev dev() const;
+
+  private:
+    int nx_, ny_, nu_, nv_;
+    DArray<double> c_;  // the 16 coefficient arrays, packed
+};
+
 // finest-level V-cycle pieces (omega-weighted Jacobi, D = diag(A)):
 //   pre:    x1 = omega D^-1 b, x2 = x1 + omega D^-1 (b - A x1), r = b - A x2
 //   sweep2: two sweeps x <- x + omega D^-1 (b - A x) (after the prolongation)

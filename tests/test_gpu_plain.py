@@ -80,6 +80,23 @@ def test_poisson_stencil_equals_csr_product(name):
 
 
 @pytest.mark.parametrize("name", ["lidDrivenCavityRe100_32", "cylinderRe40_small_luo"])
+def test_velocity_stencil_equals_csr_product(name):
+    """the velocity solve's stencil operator reproduces the assembled LHS1 bit for bit"""
+    import cuibm_amd
+    import scipy.sparse as sp
+
+    g, _ = pair(name, TIGHT)
+    rng = np.random.default_rng(12)
+    u = rng.standard_normal(g.get("uhat").size)
+    g.set("uhat", u)
+    g.phase("velocity_operator")
+    y_stencil = g.get("rhs1")
+    rp, ci, v = g.get("LHS1_rowptr"), g.get("LHS1_col"), g.get("LHS1_val")
+    A = cuibm_amd.Csr(sp.csr_matrix((v, ci, rp), shape=(u.size, u.size)))
+    assert np.array_equal(y_stencil, A.spmv(u))
+
+
+@pytest.mark.parametrize("name", ["lidDrivenCavityRe100_32", "cylinderRe40_small_luo"])
 def test_cfl_monitor_and_phase_timers(name):
     """The CFL monitor (NavierStokes/CFL.cu:10-35) keeps the largest dt (|u|/dx + |v|/dy)
     over the cells (last column and row excluded), with the first cell attaining it and
