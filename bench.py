@@ -203,6 +203,8 @@ def ensemble_bench(cu, n, steps, workers, check):
         s.close()
         log(f"[bench]   case {i} ({cases[i][0]}^2, Re {cases[i][1]:.1f}) alone in FP64: |dC_D| {dcd[-1]:.2e}")
     alone_rate = sum(cases[i][0] ** 2 for i in picked) * steps / t_alone
+    import shutil
+    shutil.rmtree(folder, ignore_errors=True)
     return {
         "metric": "aggregate cell-updates/s, ensemble of independent cylinder cases on one GPU",
         "value": round(rate, 1), "unit": "cell-updates/s", "higher_is_better": True, "n_gpus": 1,
