@@ -276,7 +276,7 @@ struct AmgDev {
                 upload_csr(h[l].R, L.R, s);
                 L.lanesP = lanes_for(h[l].P);
                 L.lanesR = lanes_for(h[l].R);
-                if (!stencil_level) {
+                if (!stencil_level && h[l].G.n > 0) {
                     upload_csr(h[l].G, L.G, s);
                     L.lanesG = lanes_for(h[l].G);
                 }
@@ -338,7 +338,12 @@ struct AmgDev {
             // post-sweeps xt -> r -> x
             AmgLevelDev<V> &C = lv[l + 1];
             launch_pre2<V>(L.lanesA, L.A, L.dinv.p(), b, L.xt.p(), w, s, stop);
-            launch_rres<V>(L.lanesG, L.G, n, L.xt.p(), b, C.b.p(), s, stop);  // R (b - A x2)
+            if (L.G.n > 0) {
+                launch_rres<V>(L.lanesG, L.G, n, L.xt.p(), b, C.b.p(), s, stop);  // R (b - A x2)
+            } else {
+                launch_csr_op<V, OP_RESID>(L.lanesA, L.A, L.xt.p(), b, nullptr, L.r.p(), 0.0, s, stop);
+                launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
+            }
             vcycle(l + 1, C.b.p(), C.x.p(), s);
             launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s, stop);
             launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), b, L.dinv.p(), L.r.p(), w, s, stop);
@@ -379,7 +384,7 @@ KernelProbe &AmgPrecond::pre_probe() { return im_->probe; }
 
 AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine)
     : im_(new Impl), prm_(p) {
-    im_->host = amg_build(A, p.coarse_max, p.max_levels, p.scale0);
+    im_->host = amg_build(A, p.coarse_max, p.max_levels, p.scale0, p.fused_restriction);
     fine_ = fine != nullptr && im_->host.size() > 1 && p.npre == 2 && p.npost == 2;
     if (fine_) {
         if (fine->nx * fine->ny != A.n) throw Error("stencil and matrix sizes differ", ERR_ARGUMENT);
@@ -499,7 +504,8 @@ double AmgPrecond::bytes_per_apply() const {
         if (prm_.npre == 2 && prm_.npost == 2) {  // the fused (2,2) sequence of vcycle()
             const double nnzG = (double)H[l].G.ci.size();
             bytes += spmv(n, n, nnz) + n * sv + n * 8;                    // both pre-sweeps (+ dinv, b)
-            bytes += nnzG * (sv + 4) + 4 * (nc + 1) + 2 * n * 8 + nc * 8; // restricted residual: G, b, x2
+            if (nnzG > 0) bytes += nnzG * (sv + 4) + 4 * (nc + 1) + 2 * n * 8 + nc * 8;  // R (b - A x2) via G
+            else bytes += spmv(n, n, nnz) + n * 8 + spmv(nc, n, nnzP);    // residual (+ b), restriction
             bytes += spmv(n, nc, nnzP) + n * 8;                           // prolong-add
             bytes += 2 * (spmv(n, n, nnz) + n * sv + n * 8);              // post-sweeps (+ dinv, b)
             continue;

@@ -122,7 +122,7 @@ int aggregate(const HCsr &A, std::vector<int> &agg) {
 
 }  // namespace
 
-std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_levels, bool scale0) {
+std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_levels, bool scale0, bool build_g) {
     std::vector<HostAmgLevel> lv;
     HCsr A = A0;
     A.ncols = A.n;
@@ -185,7 +185,7 @@ std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_leve
             for (size_t k = 0; k < L.R.v.size(); k++) L.R.v[k] = L.R.v[k] * (1.0 / d[L.R.ci[k]]);
         const HCsr AP = multiply(A, P);
         HCsr Ac = multiply(L.R, AP);
-        {  // G = [R | -R A]
+        if (build_g) {  // G = [R | -R A]
             const HCsr RA = multiply(L.R, A);
             HCsr &G = L.G;
             G.n = L.R.n;

@@ -133,6 +133,9 @@ struct AmgParams {
     // restrict the finest level with P^T D^-1 instead of P^T: the hierarchy of the
     // row-scaled D^-1 A, for matrices whose rows mix scales
     bool scale0 = false;
+    // build G = [R | -R A] and restrict residuals in one product (an extra SpGEMM in the
+    // set-up: left out for hierarchies rebuilt every step)
+    bool fused_restriction = true;
 };
 
 // HIP events around the launches of one kernel inside the live solve: mark() before
@@ -241,7 +244,8 @@ struct HostAmgLevel {
     // product with the pair (b, x), on every level but the coarsest
     HCsr G;
 };
-std::vector<HostAmgLevel> amg_build(const HCsr &A, int coarse_max, int max_levels, bool scale0 = false);
+std::vector<HostAmgLevel> amg_build(const HCsr &A, int coarse_max, int max_levels, bool scale0 = false,
+                                    bool build_g = true);
 std::vector<double> dense_inverse(const HCsr &A);
 
 // ---- BiCGSTAB (krylov.hip)

@@ -153,7 +153,7 @@ static HCsr download_csr(const DCsr<double> &A, hipStream_t s) {
 
 static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p, const DCsr<double> &A, hipStream_t s,
                                                     const P5Dev *fine = nullptr, bool scale0 = false,
-                                                    int coarse_max = 2500) {
+                                                    int coarse_max = 2500, bool fused_restriction = true) {
     switch (p.preconditioner) {
         case NONE: return std::unique_ptr<Preconditioner>(new IdentityPrecond(A.n));
         case DIAGONAL: return std::unique_ptr<Preconditioner>(new JacobiPrecond(A, s));
@@ -162,6 +162,7 @@ static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p,
             ap.precision = p.amgPrecision;
             ap.scale0 = scale0;
             ap.coarse_max = coarse_max;
+            ap.fused_restriction = fused_restriction;
             return std::unique_ptr<Preconditioner>(new AmgPrecond(download_csr(A, s), ap, s, fine));
         }
         default: throw Error("preconditioner AINV is not supported", ERR_UNSUPPORTED);
@@ -183,7 +184,7 @@ void NavierStokesSolver::buildPreconditioners() {
     }
     PC1 = make_precond(db->velocitySolve, LHS1, stream_);
     PC2 = make_precond(db->PoissonSolve, LHS2, stream_, stencilLHS2() ? &fine : nullptr, mixedScaleLHS2(),
-                       amgCoarseMax());
+                       amgCoarseMax(), !amgRebuiltEachStep());
     ks1.reset(new Bicgstab(numUV));
     ks2.reset(new Bicgstab(numP));
     CUIBM_HIP(hipStreamSynchronize(stream_));
