@@ -44,6 +44,14 @@ constexpr int EMB_ROW = 6;
 void launch_emb_lhs1(const GridDev &g, double nu, double dt, const EmbAsm &a, hipStream_t s);
 void launch_emb_lhs2(const GridDev &g, double dt, const EmbAsm &a, hipStream_t s);
 void launch_emb_compact(const EmbAsm &a, const int *rp, int *ci, double *v, hipStream_t s);
+// row pointers from per-row counts on the device (replaces the reference's single-thread
+// size_LHS kernels, luo_iter/kernels/LHS1.cu, LHS2.cu): rp[0] = 0, rp[i+1] = rp[i] + cnt[i];
+// `part` holds at least scan_parts(n) ints
+int scan_parts(int n);
+void launch_row_scan(const int *cnt, int *rp, int n, int *part, hipStream_t s);
+// row map of the embedded assembly: -1, then -(t + 2) at ghost node t, then t at hybrid
+// node t (a node in both lists is a hybrid row)
+void launch_emb_map(int *map, int n, const int *ghost, int ng, const int *hybrid, int nh, hipStream_t s);
 // rhs1 = rhs1 * ns + interp for interior velocity rows (v rows only below the thread
 // count of the u launch, as in the reference)
 void launch_emb_rhs1(double *rhs1, const double *u, const EmbAsm &a, const GridDev &g, double uB, double vB,

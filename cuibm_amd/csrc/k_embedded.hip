@@ -11,6 +11,8 @@
 // row whose tail slot is not written or written twice, an interpolation corner
 // outside the 3x3 stencil) are counted, because the reference's result then depends
 // on what the unwritten slots held from earlier steps.
+#include <algorithm>
+
 #include "embedded.hpp"
 
 namespace cuibm {
@@ -363,6 +365,103 @@ void launch_emb_rhs1(double *rhs1, const double *u, const EmbAsm &a, const GridD
 void launch_emb_rhs2(double *rhs2, const EmbAsm &a, const GridDev &g, double dudt, double dvdt, hipStream_t s) {
     hipLaunchKernelGGL(k_emb_rhs2, dim3(nblk(a.n)), dim3(BS), 0, s, rhs2, a, g.nx, g.ny, dudt, dvdt);
     CUIBM_LAUNCH_CHECK();
+}
+
+// ---- device row scan and row maps ----------------------------------------------------
+
+constexpr int SCAN_BS = 256, SCAN_PER = 8, SCAN_TILE = SCAN_BS * SCAN_PER;
+
+int scan_parts(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+// inclusive scan of one thread's values, then across the block (Hillis-Steele on LDS)
+__device__ int block_excl_scan(int v, int *sh, int &total) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_BS; o <<= 1) {
+        const int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    total = sh[SCAN_BS - 1];
+    return sh[threadIdx.x] - v;
+}
+
+// pass 1: the sum of each tile of SCAN_TILE counts
+__global__ void __launch_bounds__(SCAN_BS) k_scan_tiles(const int *__restrict__ cnt, int n, int *__restrict__ part) {
+    __shared__ int sh[SCAN_BS];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    int v = 0;
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) v += cnt[base + k];
+    int total;
+    block_excl_scan(v, sh, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// pass 2: exclusive scan of the tile sums in place (one block, looping over the tiles)
+__global__ void __launch_bounds__(SCAN_BS) k_scan_parts(int *__restrict__ part, int np) {
+    __shared__ int sh[SCAN_BS];
+    int carry = 0;
+    for (int b = 0; b < np; b += SCAN_BS) {
+        const int v = b + (int)threadIdx.x < np ? part[b + threadIdx.x] : 0;
+        int total;
+        const int e = block_excl_scan(v, sh, total);
+        if (b + (int)threadIdx.x < np) part[b + threadIdx.x] = carry + e;
+        carry += total;
+        __syncthreads();
+    }
+}
+
+// pass 3: each tile's row pointers from its offset; the last tile writes rp[n]
+__global__ void __launch_bounds__(SCAN_BS) k_scan_rows(const int *__restrict__ cnt, int n,
+                                                       const int *__restrict__ part, int *__restrict__ rp) {
+    __shared__ int sh[SCAN_BS];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    int v = 0;
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) v += cnt[base + k];
+    int total;
+    int off = part[blockIdx.x] + block_excl_scan(v, sh, total);
+    if (blockIdx.x == 0 && threadIdx.x == 0) rp[0] = 0;
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) {
+            off += cnt[base + k];
+            rp[base + k + 1] = off;
+        }
+}
+
+void launch_row_scan(const int *cnt, int *rp, int n, int *part, hipStream_t s) {
+    if (n <= 0) return;
+    const int nt = scan_parts(n) - 1;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_BS), 0, s, cnt, n, part);
+    CUIBM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(SCAN_BS), 0, s, part, nt);
+    CUIBM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_rows, dim3(nt), dim3(SCAN_BS), 0, s, cnt, n, part, rp);
+    CUIBM_LAUNCH_CHECK();
+}
+
+__global__ void k_map_fill(int *__restrict__ map, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) map[i] = -1;
+}
+__global__ void k_map_scatter(int *__restrict__ map, const int *__restrict__ node, int m, int ghost) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m; t += gridDim.x * blockDim.x)
+        map[node[t]] = ghost ? -(t + 2) : t;
+}
+
+void launch_emb_map(int *map, int n, const int *ghost, int ng, const int *hybrid, int nh, hipStream_t s) {
+    auto blocks = [](int m) { return dim3((unsigned)std::max(1, std::min((m + BS - 1) / BS, 65536))); };
+    hipLaunchKernelGGL(k_map_fill, blocks(n), dim3(BS), 0, s, map, n);
+    CUIBM_LAUNCH_CHECK();
+    if (ng > 0) {
+        hipLaunchKernelGGL(k_map_scatter, blocks(ng), dim3(BS), 0, s, map, ghost, ng, 1);
+        CUIBM_LAUNCH_CHECK();
+    }
+    if (nh > 0) {  // after the ghost scatter: a node in both lists ends as a hybrid row
+        hipLaunchKernelGGL(k_map_scatter, blocks(nh), dim3(BS), 0, s, map, hybrid, nh, 0);
+        CUIBM_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace cuibm

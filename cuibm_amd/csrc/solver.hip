@@ -1103,15 +1103,16 @@ EmbAsm luoIter::asm2() const {
 void luoIter::assemble(DCsr<double> &A, const EmbAsm &a, bool velocity, int statOff) {
     if (velocity) launch_emb_lhs1(grid(), nu, dt, a, stream_);
     else launch_emb_lhs2(grid(), dt, a, stream_);
-    std::vector<int> c(a.n);
-    cnt.download(c.data(), a.n, stream_);
-    std::vector<int> rp(a.n + 1, 0);
-    for (int i = 0; i < a.n; i++) rp[i + 1] = rp[i] + c[i];
     DCsr<double> B;
     B.n = a.n;
     B.ncols = a.n;
-    B.nnz = rp.back();
-    B.rp.upload(rp, stream_);
+    B.rp.alloc((size_t)a.n + 1);
+    scanPart.alloc(scan_parts(a.n));
+    launch_row_scan(cnt.p(), B.rp.p(), a.n, scanPart.p(), stream_);  // row pointers on the device
+    int nnz = 0;
+    CUIBM_HIP(hipMemcpyAsync(&nnz, B.rp.p() + a.n, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    CUIBM_HIP(hipStreamSynchronize(stream_));
+    B.nnz = nnz;
     B.ci.alloc(B.nnz);
     B.v.alloc(B.nnz);
     launch_emb_compact(a, B.rp.p(), B.ci.p(), B.v.p(), stream_);
@@ -1130,16 +1131,8 @@ void luoIter::initialiseLHS() {
     // a node tagged both hybrid and ghost is a hybrid row: the reference's row kernels test
     // the hybrid tag first, and its hybrid interpolation overwrites the ghost one's records
     auto make_map = [&](DArray<int> &map, int n, const Stencils &hyb, const Stencils &gho) {
-        std::vector<int> m(n, -1);
-        if (gho.n) {
-            const std::vector<int> g = gho.node.to_host(s);
-            for (int t = 0; t < gho.n; t++) m[g[t]] = -(t + 2);
-        }
-        if (hyb.n) {
-            const std::vector<int> h = hyb.node.to_host(s);
-            for (int t = 0; t < hyb.n; t++) m[h[t]] = t;
-        }
-        map.upload(m, s);
+        map.alloc(n);
+        launch_emb_map(map.p(), n, gho.n ? gho.node.p() : nullptr, gho.n, hyb.n ? hyb.node.p() : nullptr, hyb.n, s);
     };
     make_map(map1, numUV, hnV, gnV);
     make_map(map2, numP, hnP, gnP);

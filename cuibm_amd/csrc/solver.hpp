@@ -230,7 +230,7 @@ class luoIter : public luo_base {
     int amgCoarseMax() const override { return moving ? 300 : 2500; }
     bool amgRebuiltEachStep() const override { return moving; }
     // row maps (-1 plain, t hybrid slot, -(t + 2) ghost slot), row buffers, row scales
-    DArray<int> map1, map2, cnt, rowc, stats;
+    DArray<int> map1, map2, cnt, rowc, stats, scanPart;
     DArray<double> rowv, ns1, ns2;
     long embStats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     EmbAsm asm1() const;
