@@ -52,3 +52,24 @@ def test_time_step_that_does_not_converge_returns_an_error(tmp_path):
     print(msg, flush=True)
     assert e.value.code == -4
     assert "Solve for pressure failed" in msg and "Iterations   : 3" in msg
+
+
+@pytest.mark.parametrize("exact_diag", [True, False])
+def test_half_step_exit_matches_oracle(exact_diag):
+    """With a Jacobi preconditioner on a diagonal matrix, M^-1 = A^-1: the first half step
+    gives s = 0 and BiCGSTAB leaves after x += alpha M^-1 p, at the device's half-step code;
+    a nearly diagonal matrix exits later, at a half or a full step. The iterate and the
+    count must be the oracle's."""
+    n = 5000
+    rng = np.random.default_rng(9)
+    d = 1.0 + rng.random(n)
+    A = sp.diags(d).tocsr() if exact_diag else (sp.diags(d) + 1e-3 * sp.random(n, n, 2e-4, random_state=3)).tocsr()
+    A.sort_indices()
+    b = rng.standard_normal(n)
+    xg, itg, _, ok = cuibm_amd.Csr(A).bicgstab(b, precond="DIAGONAL", rtol=1e-12, maxit=50)
+    xo, ito, _ = po.OrcCsr(A).bicgstab(b, np.zeros(n), precond="DIAGONAL", rtol=1e-12, maxit=50)
+    print("exact" if exact_diag else "near", "gpu", itg, ok, "oracle", ito, "x diff", rel_l2(xg, xo), flush=True)
+    assert ok and itg == ito
+    assert rel_l2(xg, xo) < 1e-13
+    if exact_diag:
+        assert rel_l2(xg, b / d) < 1e-15
