@@ -55,10 +55,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5):
+def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5, extra=()):
     """initialise, warm up, time `steps` steps; return a dict of figures"""
     t0 = time.perf_counter()
-    s = cu.Solver(case, "-amgPrecision", str(amg_precision), diagnostics=False)
+    s = cu.Solver(case, "-amgPrecision", str(amg_precision), *extra, diagnostics=False)
     s.initialise()
     t_init = time.perf_counter() - t0
     if warmup > 0:
@@ -285,6 +285,11 @@ def main():
         log("[bench] FP64 form")
         fp64 = run_form(cu, case, 64, args.warmup, args.steps, None, 0)
         log(f"[bench] fp64: {fp64['steps_per_s']:.2f} steps/s, Poisson its/step {fp64['poisson_iters_per_step']:.1f}")
+    csr = None
+    if not args.no_fp64:
+        log("[bench] FP64 form with stored CSR operators at every level (no stencil forms)")
+        csr = run_form(cu, case, 64, args.warmup, args.steps, None, 0, extra=("-operators", "csr"))
+        log(f"[bench] fp64 csr: {csr['steps_per_s']:.2f} steps/s, Poisson its/step {csr['poisson_iters_per_step']:.1f}")
     cpu = None
     if args.cpu_steps > 0:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
@@ -347,6 +352,16 @@ def main():
         }
         line["speedup_mixed_vs_fp64"] = round(mixed["steps_per_s"] / fp64["steps_per_s"], 4)
         line["abs_dcd_mixed_vs_fp64"] = abs(mixed["cd"] - fp64["cd"])
+    if csr:
+        line["fp64_csr"] = {
+            "steps_per_s": round(csr["steps_per_s"], 4),
+            "ms_per_step": round(csr["ms_per_step"], 4),
+            "poisson_iters_per_step": csr["poisson_iters_per_step"],
+            "cd": csr["cd"],
+            "form": "FP64 hierarchy, CSR operators for both solves and every AMG level (-operators csr)",
+        }
+        line["speedup_mixed_vs_fp64_csr"] = round(mixed["steps_per_s"] / csr["steps_per_s"], 4)
+        line["abs_dcd_mixed_vs_fp64_csr"] = abs(mixed["cd"] - csr["cd"])
     print(json.dumps(line), flush=True)
 
 

@@ -296,6 +296,12 @@ void readInputs(const std::string &caseFolder, int argc, const char *const *argv
         else if (a == "-poissonTol") { DB.PoissonSolve.tolerance = std::atof(v); i++; }
         else if (a == "-amgPrecision") { DB.PoissonSolve.amgPrecision = std::atoi(v) == 32 ? 32 : 64; i++; }
         else if (a == "-poissonPreconditioner") { DB.PoissonSolve.preconditioner = preconditionerFromString(v); i++; }
+        else if (a == "-operators") {
+            const std::string o = v;
+            if (o != "csr" && o != "stencil") throw YamlError("-operators takes csr or stencil, not '" + o + "'");
+            DB.csrOperators = o == "csr";
+            i++;
+        }
     }
     if (D.nx < 3 || D.ny < 3) throw std::runtime_error("grid must have at least 3 cells in each direction");
 }

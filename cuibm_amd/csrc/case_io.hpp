@@ -39,6 +39,9 @@ struct LinearSolverParams {
 struct ParameterDB {
     std::string caseFolder = ".";
     int deviceNumber = 0;
+    // -operators csr: stored CSR operators for both solves and every AMG level, no stencil
+    // forms (no reference counterpart; the straightforward form, for comparison)
+    bool csrOperators = false;
     // flow
     double nu = 0.01, uInitial = 1.0, vInitial = 0.0, uPerturb = 0.0, vPerturb = 0.0;
     BoundaryCondition bc[4][2];

@@ -66,7 +66,7 @@ class NavierStokesSolver {
     int timeKernel(const std::string &kernel, int reps, double *ms, double *bytes);
     // LHS2 rows follow the plain formula (no IBM row overrides): the Poisson solve
     // can use the stencil form of the operator
-    virtual bool stencilLHS2() const { return true; }
+    virtual bool stencilLHS2() const { return !db->csrOperators; }
     AmgPrecond *amg2() const;  // the Poisson AMG preconditioner (throws if another)
     AmgHierarchyInfo amgInfo() const;
     // NavierStokesSolver::CFL (NavierStokes/CFL.cu:10-35): track the largest CFL number

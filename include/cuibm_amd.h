@@ -46,7 +46,7 @@ const char *cuibm_last_error(void);
 /* io::readInputs (src/io/io.cu:108-138) + the SolverType factory (src/cuIBM.cu:42-60):
  * read <case_folder>/{simParams,flow,domain,bodies}.yaml, apply the command-line
  * overrides in argv (-nu -uPerturb -vPerturb -scaleCV -nsave -nt -dt -velocityTol
- * -poissonTol, plus -amgPrecision 32|64, -poissonPreconditioner NAME,
+ * -poissonTol, plus -amgPrecision 32|64, -poissonPreconditioner NAME, -operators csr|stencil,
  * -outputFolder DIR and -output 0|1) and construct the solver (not yet initialised). */
 int cuibm_create(const char *case_folder, int argc, const char *const *argv, cuibm_solver **out);
 

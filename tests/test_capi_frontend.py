@@ -105,3 +105,11 @@ def test_front_end_on_reference_domains(c, tmp_path):
         assert np.array_equal(s.case_array("body_X"), X)
         assert np.array_equal(s.case_array("body_Y"), Y)
     s.close()
+
+
+def test_operators_switch_is_parsed():
+    s = cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-operators", "csr")
+    s.close()
+    with pytest.raises(cuibm_amd.CuibmError) as e:
+        cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-operators", "dense")
+    assert e.value.code == -2 and "csr or stencil" in str(e.value)

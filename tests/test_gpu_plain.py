@@ -124,3 +124,24 @@ def test_cfl_monitor_and_phase_timers(name):
     for e in ("initialise", "Intermediate Velocity Setup", "Intermediate Velocity Solve", "Poisson Setup",
               "Poisson Solve", "Velocity Projection", "CFL"):
         assert e in t and t[e] >= 0, e
+
+
+def test_csr_operator_form_matches_stencil_form():
+    """-operators csr (stored CSR operators for both solves and every AMG level) against the
+    default stencil forms on the Luo cylinder: the operators are the same numbers, only the
+    V-cycle's finest-level arithmetic order differs, so u and C_D agree to rounding"""
+    import cuibm_amd
+    from helpers import case
+
+    forms = []
+    for extra in ((), ("-operators", "csr")):
+        g = cuibm_amd.Solver(case("cylinderRe40_small_luo"), "-poissonTol", "1e-12", "-velocityTol", "1e-12", *extra,
+                             diagnostics=False)
+        g.initialise()
+        g.step_time(5)
+        forms.append((g.get("u"), 2.0 * g.info()["forceX"]))
+        g.close()
+    (u0, cd0), (u1, cd1) = forms
+    print("u", np.linalg.norm(u1 - u0) / np.linalg.norm(u0), "C_D", cd0, cd1, flush=True)
+    assert np.linalg.norm(u1 - u0) <= 1e-10 * np.linalg.norm(u0)
+    assert abs(cd1 - cd0) <= 1e-9
