@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only.
 //
 // This is the CPU (FP64, OpenMP) restatement of cuIBM's per-time-step hot path
-// that the parity tests and the bench's cpu_baseline leg use as the checker.
+// that the parity tests and the bench's cpu_baseline leg compare against.
 // Nothing in the product (cuibm_amd/) links, loads or calls it.  Every function
 // cites the reference file:line it restates; the reference itself (CUDA + CUSP
 // 0.5.1) cannot be compiled here, so this restatement is the oracle.

@@ -2,7 +2,7 @@
 
 ctypes front end of ``liboracle.so`` (the FP64 CPU restatement of cuIBM's hot
 path, see ``orc.hpp``).  Only the tests, ``__graft_entry__.smoke()`` and the
-bench's ``cpu_baseline`` leg import this module, and only as the checker.
+bench's ``cpu_baseline`` leg import this module, and only to compare against it.
 
 Case files are read with PyYAML here (the product has its own C++ reader); the
 keys and defaults follow src/io/parseSimulationFile.cu:71-182,
