@@ -101,3 +101,19 @@ def test_viv_matches_oracle(name, steps):
         for f in ("forceX", "forceY"):
             assert abs(gi[f] - oi[f]) <= 1e-8 * max(1.0, abs(oi[f])), (k, f, gi[f], oi[f])
     assert g.info()["midY"] != 0
+
+
+def test_cylinder_336_luo_60_steps_as_configured():
+    """The cfg2 cylinder (336x336, Re=40, Luo external) at its own tolerances (1e-8): no drift
+    between the GPU and the oracle over 60 steps of the impulsive start, checked every 20
+    steps: u within 1e-10, drag and lift within 1e-8 relative."""
+    g, o = pair("cylinderRe40_luo")
+    for k in range(3):
+        g.step_time(20)
+        o.step(20)
+        gi, oi = g.info(), o.info()
+        du = rel_l2(g.get("u"), o.get("u"))
+        print(f"step {20 * (k + 1)}: u {du:.2e} C_D {2 * gi['forceX']:.12f} {2 * oi['forceX']:.12f}", flush=True)
+        assert du < 1e-10, k
+        for f in ("forceX", "forceY"):
+            assert abs(gi[f] - oi[f]) <= 1e-8 * max(1.0, abs(oi[f])), (k, f, gi[f], oi[f])
