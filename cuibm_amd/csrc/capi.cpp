@@ -303,6 +303,9 @@ int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms,
 int cuibm_csr_create(int n, const int *rowptr, const int *col, const double *val, cuibm_csr **out) {
     return guard([&] {
         if (!out || n <= 0 || !rowptr || !col || !val) throw Error("bad CSR arguments", ERR_ARGUMENT);
+        if (rowptr[0] != 0) throw Error("CSR row pointer must start at 0", ERR_ARGUMENT);
+        for (int i = 0; i < n; i++)
+            if (rowptr[i + 1] < rowptr[i]) throw Error("CSR row pointer decreases at row " + std::to_string(i), ERR_ARGUMENT);
         std::unique_ptr<cuibm_csr> h(new cuibm_csr);
         CUIBM_HIP(hipStreamCreate(&h->stream));
         h->h.n = n;

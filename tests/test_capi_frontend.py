@@ -113,3 +113,20 @@ def test_operators_switch_is_parsed():
     with pytest.raises(cuibm_amd.CuibmError) as e:
         cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-operators", "dense")
     assert e.value.code == -2 and "csr or stencil" in str(e.value)
+
+
+def test_csr_create_rejects_a_malformed_row_pointer():
+    """row pointers that do not start at 0 or that decrease are refused before any upload"""
+    import ctypes as C
+
+    lib = cuibm_amd.lib()
+    IP = C.POINTER(C.c_int)
+    ci = np.zeros(4, dtype=np.int32)
+    v = np.ones(4)
+    h = C.c_void_p()
+    for rp in ([1, 2, 3], [0, 2, 1], [0, 3, 2]):
+        rp = np.array(rp, dtype=np.int32)
+        rc = lib.cuibm_csr_create(2, rp.ctypes.data_as(IP), ci.ctypes.data_as(IP),
+                                  v.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h))
+        assert rc == -6, rp
+        assert b"row pointer" in lib.cuibm_last_error()
