@@ -792,7 +792,8 @@ __global__ void __launch_bounds__(BS) k_force(const double *__restrict__ u, cons
         idx -= n0;
         if (idx < n1) {  // dragBottomTop
             const int Iu = J * (nx - 1) + (I - 1 + idx), Iv = (nx - 1) * ny + (J - 1) * nx + I + idx;
-            acc[1] += -((0.25 * (u[Iu + ncy * (nx - 1)] + u[Iu + (ncy - 1) * (nx - 1)]) - 0.25 * (u[Iu] + u[Iu - (nx - 1)])) -
+            acc[1] += -((0.25 * (u[Iu + ncy * (nx - 1)] + u[Iu + (ncy - 1) * (nx - 1)]) * (u[Iv + ncy * nx] + u[Iv + ncy * nx - 1]) -
+                          0.25 * (u[Iu] + u[Iu - (nx - 1)]) * (u[Iv] + u[Iv - 1])) -
                         nu * (((u[Iu + ncy * (nx - 1)] - u[Iu + (ncy - 1) * (nx - 1)]) / 2.0 / (dy[J + ncy] + dy[J + ncy - 1]) +
                                (u[Iv + ncy * nx] - u[Iv + ncy * nx - 1]) / 2.0 / (dx[I + idx] + dx[I + idx - 1])) -
                               ((u[Iu] - u[Iu - (nx - 1)]) / 2.0 / (dy[J] + dy[J - 1]) +
@@ -810,7 +811,8 @@ __global__ void __launch_bounds__(BS) k_force(const double *__restrict__ u, cons
         idx -= n2;
         if (idx < n3) {  // liftLeftRight
             const int Iu = (J + idx) * (nx - 1) + (I - 1), Iv = (nx - 1) * ny + (J - 1 + idx) * nx + I;
-            acc[3] += -((0.25 * (u[Iu + ncx] + u[Iu + ncx - (nx - 1)]) - 0.25 * (u[Iu] + u[Iu - (nx - 1)])) -
+            acc[3] += -((0.25 * (u[Iu + ncx] + u[Iu + ncx - (nx - 1)]) * (u[Iv + ncx] + u[Iv + ncx - 1]) -
+                          0.25 * (u[Iu] + u[Iu - (nx - 1)]) * (u[Iv] + u[Iv - 1])) -
                         nu * (((u[Iu + ncx] - u[Iu + ncx - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +
                                (u[Iv + ncx] - u[Iv + ncx - 1]) / 2.0 / (dx[I + ncx] + dx[I + ncx - 1])) -
                               ((u[Iu] - u[Iu - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +

@@ -826,7 +826,8 @@ void Solver::force() {
     }
     for (int idx = 0; idx <= ncx; idx++) {  // dragBottomTop
         const int Iu = J * (nx - 1) + (I - 1 + idx), Iv = (nx - 1) * ny + (J - 1) * nx + I + idx;
-        sy += -((0.25 * (uu[Iu + ncy * (nx - 1)] + uu[Iu + (ncy - 1) * (nx - 1)]) - 0.25 * (uu[Iu] + uu[Iu - (nx - 1)]))
+        sy += -((0.25 * (uu[Iu + ncy * (nx - 1)] + uu[Iu + (ncy - 1) * (nx - 1)]) * (uu[Iv + ncy * nx] + uu[Iv + ncy * nx - 1]) -
+                          0.25 * (uu[Iu] + uu[Iu - (nx - 1)]) * (uu[Iv] + uu[Iv - 1]))
                 - nu * (((uu[Iu + ncy * (nx - 1)] - uu[Iu + (ncy - 1) * (nx - 1)]) / 2.0 / (dy[J + ncy] + dy[J + ncy - 1]) +
                          (uu[Iv + ncy * nx] - uu[Iv + ncy * nx - 1]) / 2.0 / (dx[I + idx] + dx[I + idx - 1])) -
                         ((uu[Iu] - uu[Iu - (nx - 1)]) / 2.0 / (dy[J] + dy[J - 1]) +
@@ -845,7 +846,8 @@ void Solver::force() {
     double lx = 0, ly = 0, lu = 0;
     for (int idx = 0; idx <= ncy; idx++) {  // liftLeftRight
         const int Iu = (J + idx) * (nx - 1) + (I - 1), Iv = (nx - 1) * ny + (J - 1 + idx) * nx + I;
-        lx += -((0.25 * (uu[Iu + ncx] + uu[Iu + ncx - (nx - 1)]) - 0.25 * (uu[Iu] + uu[Iu - (nx - 1)]))
+        lx += -((0.25 * (uu[Iu + ncx] + uu[Iu + ncx - (nx - 1)]) * (uu[Iv + ncx] + uu[Iv + ncx - 1]) -
+                          0.25 * (uu[Iu] + uu[Iu - (nx - 1)]) * (uu[Iv] + uu[Iv - 1]))
                 - nu * (((uu[Iu + ncx] - uu[Iu + ncx - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +
                          (uu[Iv + ncx] - uu[Iv + ncx - 1]) / 2.0 / (dx[I + ncx] + dx[I + ncx - 1])) -
                         ((uu[Iu] - uu[Iu - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +
