@@ -206,3 +206,7 @@ struct Solver {
 };
 
 }  // namespace orc
+
+// one transverse spring-mass step of a VIV body (luo_base/move.cu:38-93)
+extern "C" void orc_viv_update(double yold, double vold, double Cy, int Ured, double dt, int strong, double ycur,
+                               double *vnew, double *ynew);

@@ -232,19 +232,9 @@ void Solver::move_body() {
 // response to the lift, reduced mass 2; the strongly coupled form relaxes the position
 // and reports its relative change in sc_tol
 void Solver::viv_move(bool strong) {
-    const double Cy = B.forceY * 2.0, Mred = 2.0;
-    const int Ured = P.ured;
-    const double vold = B.centerV0, yold = B.midY0, dt = P.dt;
-    const double a = dt * M_PI * M_PI * 4 / (Ured * Ured), b = dt * dt * 2 * M_PI * M_PI / (Ured * Ured);
-    const double vnew = (vold - a * (yold + dt / 2 * vold) + dt * Cy / 2 / Mred) / (1 + b);
-    double ynew;
-    if (strong) {
-        const double relax = 0.9;
-        sc_tol = B.midY;
-        ynew = relax * (yold + dt / 2 * (vnew + vold)) + (1 - relax) * B.midY;
-    } else {
-        ynew = yold + dt / 2 * (vnew + vold);
-    }
+    double vnew, ynew;
+    sc_tol = B.midY;
+    orc_viv_update(B.midY0, B.centerV0, B.forceY * 2.0, P.ured, P.dt, strong ? 1 : 0, B.midY, &vnew, &ynew);
     B.centerV = vnew;
     B.midY = ynew;
     for (int i = 0; i < B.n; i++) {
@@ -608,4 +598,23 @@ int orc_amg_prolongator_dense(void *A, double *out, int cap) {
         for (int k = P.rp[i]; k < P.rp[i + 1]; k++) out[(size_t)i * n1 + P.ci[k]] += P.v[k];
     return n1;
 }
+}
+
+// luo_base/move.cu:38-93 (viv_movement_LC / viv_movement_SC): one step of the transverse
+// spring-mass response to the lift coefficient Cy, reduced mass 2 and integer reduced
+// velocity Ured: the velocity from the spring force at the new position, the position by
+// the trapezoidal rule; the strongly coupled form relaxes the position by 0.9 towards
+// ycur, the position of the previous pass
+extern "C" void orc_viv_update(double yold, double vold, double Cy, int Ured, double dt, int strong, double ycur,
+                               double *vnew, double *ynew) {
+    const double Mred = 2.0;
+    const double a = dt * M_PI * M_PI * 4 / (Ured * Ured), b = dt * dt * 2 * M_PI * M_PI / (Ured * Ured);
+    const double v = (vold - a * (yold + dt / 2 * vold) + dt * Cy / 2 / Mred) / (1 + b);
+    *vnew = v;
+    if (strong) {
+        const double relax = 0.9;
+        *ynew = relax * (yold + dt / 2 * (v + vold)) + (1 - relax) * ycur;
+    } else {
+        *ynew = yold + dt / 2 * (v + vold);
+    }
 }

@@ -88,6 +88,9 @@ def lib():
         L.orc_amg_coarse_dense.argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
         L.orc_amg_prolongator_dense.restype = C.c_int
         L.orc_amg_prolongator_dense.argtypes = [C.c_void_p, _DP, C.c_int]
+        L.orc_viv_update.restype = None
+        L.orc_viv_update.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
+                                     _DP, _DP]
         L.orc_embedded_stats.restype = None
         L.orc_embedded_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
         _lib = L
@@ -183,6 +186,13 @@ def read_case(folder, overrides=None):
         else:
             sim[k] = v
     return case
+
+
+def viv_update(yold, vold, cy, ured, dt, strong=False, ycur=0.0):
+    """one spring-mass step of a VIV body (luo_base/move.cu:38-93): (vnew, ynew)"""
+    v, y = C.c_double(0), C.c_double(0)
+    lib().orc_viv_update(yold, vold, cy, int(ured), dt, int(bool(strong)), ycur, C.byref(v), C.byref(y))
+    return v.value, y.value
 
 
 def set_amg_scale0(on):

@@ -400,3 +400,134 @@ def test_control_volume_force_does_not_depend_on_the_box():
     print("C_D", cds, "C_L", cls)
     assert (max(cds) - min(cds)) / np.mean(cds) < 0.04
     assert max(abs(c) for c in cls) < 0.02 * np.mean(cds)
+
+
+# ----------------------------------------------------------------------------- VIV
+
+@pytest.mark.parametrize("ured", [3, 4, 6])
+def test_viv_update_is_the_spring_mass_ode(ured):
+    """The VIV body (luo_base/move.cu:38-93) follows y'' + w^2 y = C_L / (2 m*) with reduced
+    mass m* = 2 and natural frequency w = 2 pi / Ured (U = D = 1): under a constant lift the
+    damped discrete motion settles at the static deflection C_L / (2 m* w^2); released from
+    y = 1 at rest with no lift it oscillates with period Ured (to O((w dt)^2)) and loses
+    amplitude at the first-order rate of its implicit spring force, less than 2 pi w dt per
+    period, never gaining any. A factor pi for 2 pi, Ured for Ured^2, a sign of the spring
+    force or a dropped half-step term breaks one of them."""
+    dt, w = 0.01, 2 * np.pi / ured
+    y, v = 0.0, 0.0
+    for _ in range(50000):  # a larger step damps faster; the equilibrium does not depend on it
+        v, y = po.viv_update(y, v, 0.8, ured, 0.05)
+    assert abs(y - 0.8 / (2 * 2.0 * w * w)) < 1e-12 and abs(v) < 1e-12
+    y, v = 1.0, 0.0
+    ys = [y]
+    for _ in range(int(2.5 * ured / dt)):
+        v, y = po.viv_update(y, v, 0.0, ured, dt)
+        ys.append(y)
+    ys = np.array(ys)
+    t = dt * np.arange(len(ys))
+    up = np.where((ys[:-1] < 0) & (ys[1:] >= 0))[0]  # upward zero crossings
+    tz = t[up] - ys[up] * dt / (ys[up + 1] - ys[up])
+    period = np.diff(tz)
+    assert len(period) >= 1 and np.all(np.abs(period / ured - 1) < 2 * (w * dt) ** 2)
+    n1 = int(round(ured / dt))
+    assert 1 - 2 * np.pi * w * dt < ys[n1] < 1
+    # the strongly coupled form relaxes the new position by 0.9 towards the last pass's
+    vs, ys_ = po.viv_update(0.2, 0.1, 0.5, ured, dt, strong=True, ycur=0.3)
+    vl, yl = po.viv_update(0.2, 0.1, 0.5, ured, dt)
+    assert vs == vl and abs(ys_ - (0.9 * yl + 0.1 * 0.3)) < 1e-15
+
+
+# ----------------------------------------------------------------------------- modified Fadlun
+
+@pytest.mark.parametrize("square", [False, True])
+def test_fadlun_lhs2_rows_sum_to_zero_except_the_pins(square):
+    """Cut-cell LHS2 (FadlunModified/kernels/LHS2.cu:12-248): every row's diagonal is the sum
+    of its couplings, cut distances or not, so constants are in the null space except at the
+    two pinned rows, cell 0 and the centre cell (both diagonals doubled, :244-247)"""
+    c = _square_case("FADLUN") if square else po.read_case(case("cylinderRe40_small_fadlun"))
+    c["sim"]["Poisson"]["preconditioner"] = "DIAGONAL"
+    o = po.Oracle(c)
+    A = _csr(o, "LHS2")
+    nx, ny = o.nx, o.ny
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    big = np.where(np.abs(rs) > 1e-12 * np.abs(A.diagonal()).max())[0]
+    assert list(big) == [0, (ny // 2) * nx + nx // 2]
+    # each pinned row sums to half its diagonal
+    for r in big:
+        assert abs(rs[r] - 0.5 * A[r, r]) < 1e-12 * abs(A[r, r])
+
+
+def test_fadlun_velocity_rows_reproduce_a_linear_profile_at_a_flat_wall():
+    """Modified Fadlun velocity rows (FadlunModified/kernels/LHS1.cu:12-484 with the
+    right-hand side of FadlunModified/kernels/intermediateVelocity.cu:44-63, 0 for a static
+    body): a hybrid node is the linear interpolation between its body intercept and the next
+    node out, a ghost node the linear extrapolation through the intercept from the next node
+    in. Both are exact for a velocity linear in the wall-normal coordinate that vanishes on
+    the wall, at the top (y-rays) and left (x-rays) edges of a square body. The ghost rule
+    uses dx for both directions (the reference's note: uniform dx = dy near the body), so the
+    grid here has equal core spacings."""
+    c = _square_case("FADLUN")
+    c["grid"]["y"] = po.axis(-15.0, 0.1, [-0.6, 0.6, 15.0], [1.1, 1.0, 1.1])
+    c["grid"]["mid_h"] = 0.1
+    o = po.Oracle(c)
+    x0, x1, y0, y1 = _edges(o)
+    x, y, dx, dy, xu, yv, _ = _coords(o)
+    A = _csr(o, "LHS1")
+    XU, YU = np.meshgrid(xu, y)
+    XV, YV = np.meshgrid(x, yv)
+    X = np.concatenate([XU.ravel(), XV.ravel()])
+    Y = np.concatenate([YU.ravel(), YV.ravel()])
+    g, h = o.get("ghostTagsUV"), o.get("hybridTagsUV")
+    for u, near in ((1.3 * (Y - y1), (np.abs(X - 0.5 * (x0 + x1)) < 0.15) & (Y > y1 - 0.25)),
+                    (0.7 * (x0 - X), (np.abs(Y - 0.5 * (y0 + y1)) < 0.15) & (X < x0 + 0.25))):
+        rows = np.where(((g > 0) | (h >= 0)) & near)[0]
+        assert (g[rows] > 0).sum() >= 4 and (h[rows] >= 0).sum() >= 4
+        r = A @ u
+        assert np.max(np.abs(r[rows])) < 1e-14 * np.max(np.abs(u[rows]))
+
+
+def test_fadlun_projection_is_divergence_free_in_the_fluid():
+    """The modified Fadlun pressure step is consistent: RHS2 (FadlunModified/kernels/
+    intermediatePressure.cu:17-138) is minus the divergence the cut-cell LHS2 is built from,
+    and the projection (FadlunModified/kernels/projectVelocity.cu:14-44) corrects only faces
+    between two fluid cells. So after an exact solve of LHS2 p = rhs2 from an arbitrary uhat
+    and the projection, the same divergence of the projected velocity vanishes in every
+    fluid cell (hybrid cells with cut distances included) except the pinned ones, and every
+    face touching the body keeps uhat. The fluid cells decouple from the cells inside the
+    body, so the fluid block is solved alone. The hybrid rows use the cell width for the
+    distance to every neighbour (a uniform grid around the body), so the body (radius 0.3)
+    sits well inside the uniform core [-0.6, 0.6] of the grid."""
+    c = po.read_case(case("cylinderRe40_small_fadlun"))
+    c["sim"]["Poisson"]["preconditioner"] = "DIAGONAL"
+    c["bodies"] = [po.circle(0.0, 0.0, 0.3, 40, False, 0.0)]
+    o = po.Oracle(c)
+    nx, ny = o.nx, o.ny
+    A = _csr(o, "LHS2")
+    rng = np.random.default_rng(8)
+    uhat = rng.standard_normal(o.get("u").size)
+    o.set("uhat", uhat)
+    o.phase("fadlun_rhs2")
+    b = o.get("rhs2")
+    gp, hp = o.get("ghostTagsP"), o.get("hybridTagsP")
+    keep = np.where(gp == -1)[0]
+    assert abs(A[keep][:, np.where(gp != -1)[0]]).max() == 0
+    p = np.zeros_like(b)
+    p[keep] = spla.spsolve(A[keep][:, keep].tocsc(), b[keep])
+    o.set("p", p)
+    o.phase("fadlun_project")
+    u = o.get("u")
+    o.set("uhat", u)
+    o.phase("fadlun_rhs2")
+    div = o.get("rhs2")
+    fluid = np.where(gp == -1)[0]
+    fluid = fluid[(fluid != 0) & (fluid != (ny // 2) * nx + nx // 2)]
+    assert (hp[fluid] >= 0).sum() > 10
+    assert np.max(np.abs(div[fluid])) < 1e-11 * np.max(np.abs(b))
+    # faces with a non-fluid cell on either side keep uhat (u faces, then v faces)
+    numU = (nx - 1) * ny
+    G = gp.reshape(ny, nx)
+    mu = ((G[:, :-1] != -1) | (G[:, 1:] != -1)).ravel()
+    mv = ((G[:-1, :] != -1) | (G[1:, :] != -1)).ravel()
+    touched = np.concatenate([mu, mv])
+    assert touched.sum() > 10 and np.array_equal(u[touched], uhat[touched])
+    assert not np.array_equal(u[~touched], uhat[~touched])
