@@ -99,6 +99,7 @@ def lib():
     L.cuibm_csr_destroy.restype = None
     L.cuibm_csr_spmv.argtypes = [C.c_void_p, _DP, _DP]
     L.cuibm_bicgstab.argtypes = [C.c_void_p, _DP, _DP, C.c_int, C.c_int, C.c_double, C.c_int, _IP, _DP]
+    L.cuibm_bicgstab_noalloc.argtypes = [C.c_void_p, _DP, _DP, C.c_int, C.c_double, C.c_int, _IP, _DP]
     L.cuibm_amg_levels.argtypes = [C.c_void_p, _IP, _IP, C.c_int]
     L.cuibm_amg_vcycle.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
     L.cuibm_ensemble_run.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int,
@@ -290,6 +291,19 @@ class Csr:
         r = C.c_double(0)
         rc = lib().cuibm_bicgstab(self._h, _dp(x), _dp(b), PRECONDITIONERS[precond], amg_precision, rtol, maxit,
                                   C.byref(it), C.byref(r))
+        if rc < 0 and rc != -4:
+            _check(rc)
+        return x, it.value, r.value, rc == 0
+
+    def bicgstab_noalloc(self, b, x0=None, precond="NONE", rtol=1e-10, maxit=1000):
+        """the same solve as bicgstab() (NONE or DIAGONAL) with every device buffer in the
+        library's fixed scratch array: no device allocation, n <= 2**18"""
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        it = C.c_int(0)
+        r = C.c_double(0)
+        rc = lib().cuibm_bicgstab_noalloc(self._h, _dp(x), _dp(b), PRECONDITIONERS[precond], rtol, maxit,
+                                          C.byref(it), C.byref(r))
         if rc < 0 and rc != -4:
             _check(rc)
         return x, it.value, r.value, rc == 0

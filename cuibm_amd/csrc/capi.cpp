@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/cuibm_amd.h"
+#include "noalloc.hpp"
 #include "solver.hpp"
 #include "yaml_lite.hpp"
 
@@ -363,6 +364,17 @@ int cuibm_bicgstab(cuibm_csr *A, double *x, const double *b, int precond, int am
         Bicgstab ks(n);
         const SolveStat st = ks.solve(CsrOp(A->A), dx.p(), db.p(), *M, rtol, maxit, A->stream);
         dx.download(x, n, A->stream);
+        if (iters) *iters = st.iterations;
+        if (resid) *resid = st.residual;
+        return st.converged ? 0 : (int)ERR_NOT_CONVERGED;
+    });
+}
+
+int cuibm_bicgstab_noalloc(cuibm_csr *A, double *x, const double *b, int precond, double rtol, int maxit, int *iters,
+                           double *resid) {
+    return guard([&] {
+        if (!A || !x || !b) throw Error("null matrix or vector", ERR_ARGUMENT);
+        const SolveStat st = bicgstab_noalloc(A->A, x, b, precond, rtol, maxit, A->stream);
         if (iters) *iters = st.iterations;
         if (resid) *resid = st.residual;
         return st.converged ? 0 : (int)ERR_NOT_CONVERGED;

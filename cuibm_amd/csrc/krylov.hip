@@ -21,30 +21,52 @@
 namespace cuibm {
 
 JacobiPrecond::JacobiPrecond(const DCsr<double> &A, hipStream_t s) : n_(A.n) {
-    dinv_.alloc(A.n);
-    launch_diag_inv(A, dinv_.p(), s);
+    own_.alloc(A.n);
+    dinv_ = own_.p();
+    launch_diag_inv(A, dinv_, s);
+}
+
+JacobiPrecond::JacobiPrecond(const DCsr<double> &A, double *dinv, hipStream_t s) : n_(A.n), dinv_(dinv) {
+    launch_diag_inv(A, dinv_, s);
 }
 
 void JacobiPrecond::apply(const double *r, double *z, hipStream_t s) {
-    launch_jacobi_apply(n_, dinv_.p(), r, z, s, stop_);
+    launch_jacobi_apply(n_, dinv_, r, z, s, stop_);
 }
 
 Bicgstab::Bicgstab(int n) : n_(n) {
-    for (auto *a : {&r_, &rs_, &p_, &Mp_, &AMp_, &s_, &Ms_, &AMs_}) a->alloc(n);
-    part_.alloc(2 * RED_GRID);
-    sc_.alloc(SC_COUNT);
+    own_.alloc(workspace_doubles(n));
+    double *ws = own_.p();
+    const size_t st = stride(n);
+    for (double **a : {&r_, &rs_, &p_, &Mp_, &AMp_, &s_, &Ms_, &AMs_}) {
+        *a = ws;
+        ws += st;
+    }
+    part_ = ws;
+    sc_ = ws + 2 * RED_GRID;
     CUIBM_HIP(hipHostMalloc((void **)&host_sc_, SC_COUNT * sizeof(double), hipHostMallocDefault));
+    pinned_ = true;
+}
+
+Bicgstab::Bicgstab(int n, double *ws, double *host_sc) : n_(n), host_sc_(host_sc) {
+    const size_t st = stride(n);
+    for (double **a : {&r_, &rs_, &p_, &Mp_, &AMp_, &s_, &Ms_, &AMs_}) {
+        *a = ws;
+        ws += st;
+    }
+    part_ = ws;
+    sc_ = ws + 2 * RED_GRID;
 }
 
 Bicgstab::~Bicgstab() {
-    if (host_sc_) (void)hipHostFree(host_sc_);
+    if (pinned_ && host_sc_) (void)hipHostFree(host_sc_);
 }
 
 SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditioner &M, double rtol,
                           int maxit, hipStream_t s) {
     if (A.n() != n_) throw Error("BiCGSTAB workspace size mismatch", ERR_ARGUMENT);
     const int n = n_;
-    double *sc = sc_.p(), *part = part_.p();
+    double *sc = sc_, *part = part_;
     const double *stop = sc + SC_STOP;
     A.set_stop(stop);
     M.set_stop(stop);
@@ -54,7 +76,7 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
         CUIBM_HIP(hipStreamSynchronize(s));
     };
     auto start = [&]() {  // r = b - A x, rs = p = r; clears the stop code
-        A.init(x, b, r_.p(), p_.p(), rs_.p(), part, s);
+        A.init(x, b, r_, p_, rs_, part, s);
         launch_bicg_finish(part, FIN_INIT, sc, s);
         fetch();
     };
@@ -67,19 +89,19 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
     st.tolerance = tol;
     // the first half of iteration k (through the test on ||s||), then the second half
     auto half1 = [&]() {
-        M.apply(p_.p(), Mp_.p(), s);
-        A.spmv_dot(Mp_.p(), AMp_.p(), rs_.p(), false, part, s);
+        M.apply(p_, Mp_, s);
+        A.spmv_dot(Mp_, AMp_, rs_, false, part, s);
         launch_bicg_finish(part, FIN_ALPHA, sc, s);
-        launch_bicg_s(n, r_.p(), AMp_.p(), s_.p(), sc, part, s);
+        launch_bicg_s(n, r_, AMp_, s_, sc, part, s);
         launch_bicg_finish(part, FIN_SS, sc, s, tol, maxit);
     };
     auto half2 = [&](int k) {
-        M.apply(s_.p(), Ms_.p(), s);
-        A.spmv_dot(Ms_.p(), AMs_.p(), s_.p(), true, part, s);
+        M.apply(s_, Ms_, s);
+        A.spmv_dot(Ms_, AMs_, s_, true, part, s);
         launch_bicg_finish(part, FIN_OMEGA, sc, s);
-        launch_bicg_xr(n, x, Mp_.p(), Ms_.p(), r_.p(), s_.p(), AMs_.p(), rs_.p(), sc, part, k, s);
+        launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, k, s);
         launch_bicg_finish(part, FIN_BETA, sc, s, tol, maxit);
-        launch_bicg_p(n, p_.p(), r_.p(), AMp_.p(), sc, s);
+        launch_bicg_p(n, p_, r_, AMp_, sc, s);
     };
     while (!(rn <= tol) && it < maxit) {
         // iterations to queue before the next read-back: a fraction of the estimate from
@@ -100,7 +122,7 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
             if (exact_) {  // no launch past a half-step exit or breakdown
                 fetch();
                 if (host_sc_[SC_STOP] == STOP_HALF)  // the half-step update alone
-                    launch_bicg_xr(n, x, Mp_.p(), Ms_.p(), r_.p(), s_.p(), AMs_.p(), rs_.p(), sc, part, it + j, s);
+                    launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, it + j, s);
                 if (host_sc_[SC_STOP] != 0.0) break;
             }
             half2(it + j);

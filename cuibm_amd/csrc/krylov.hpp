@@ -114,13 +114,16 @@ class IdentityPrecond : public Preconditioner {
 class JacobiPrecond : public Preconditioner {
   public:
     JacobiPrecond(const DCsr<double> &A, hipStream_t s);
+    // D^-1 kept in caller-owned device memory of A.n doubles (allocates nothing)
+    JacobiPrecond(const DCsr<double> &A, double *dinv, hipStream_t s);
     void apply(const double *r, double *z, hipStream_t s) override;
     std::string name() const override { return "DIAGONAL"; }
     double bytes_per_apply() const override { return 24.0 * n_; }
 
   private:
     int n_;
-    DArray<double> dinv_;
+    DArray<double> own_;
+    double *dinv_ = nullptr;
 };
 
 // smoothed-aggregation AMG (setup on the host, V-cycle on the device)
@@ -259,7 +262,11 @@ struct SolveStat {
 class Bicgstab {
   public:
     explicit Bicgstab(int n);
+    // workspace in caller-owned memory: `ws` holds workspace_doubles(n) device doubles
+    // (256-byte aligned), `host_sc` SC_COUNT host doubles; allocates nothing
+    Bicgstab(int n, double *ws, double *host_sc);
     ~Bicgstab();
+    static size_t workspace_doubles(int n) { return 8 * stride(n) + 2 * RED_GRID + SC_COUNT; }
     // cusp::krylov::bicgstab + cusp::monitor(b, maxit, rtol) semantics; x is the warm start
     SolveStat solve(const LinOp &A, double *x, const double *b, Preconditioner &M, double rtol, int maxit,
                     hipStream_t s);
@@ -272,11 +279,16 @@ class Bicgstab {
 
   private:
     static constexpr int kMaxRestarts = 64;
+    // vector stride in the external workspace: a multiple of 32 doubles (256 bytes)
+    static size_t stride(int n) { return ((size_t)n + 31) / 32 * 32; }
     int n_;
     bool exact_ = false;
     double rate_ = 0;  // mean residual reduction per iteration of the last solve
-    DArray<double> r_, rs_, p_, Mp_, AMp_, s_, Ms_, AMs_, part_, sc_;
+    DArray<double> own_;
+    double *r_ = nullptr, *rs_ = nullptr, *p_ = nullptr, *Mp_ = nullptr, *AMp_ = nullptr, *s_ = nullptr,
+           *Ms_ = nullptr, *AMs_ = nullptr, *part_ = nullptr, *sc_ = nullptr;
     double *host_sc_ = nullptr;
+    bool pinned_ = false;  // host_sc_ is this object's pinned allocation
 };
 
 }  // namespace cuibm

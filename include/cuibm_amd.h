@@ -197,6 +197,19 @@ int cuibm_csr_spmv(cuibm_csr *A, const double *x, double *y);
  * CUIBM_ERR_NOT_CONVERGED otherwise, with *iters and *resid set either way. */
 int cuibm_bicgstab(cuibm_csr *A, double *x, const double *b, int precond, int amg_precision, double rtol, int maxit,
                    int *iters, double *resid);
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same solve as cuibm_bicgstab with precond 0 (NONE) or 1 (DIAGONAL), whose device
+ * buffers (the staged x and b, the Jacobi diagonal, the eight BiCGSTAB vectors, the
+ * reduction partials and the scalar block) all live in one library-owned device array
+ * of 11 * 2^18 + 4112 doubles (23.1 MB), one per GPU, held until the process ends.
+ * Allocates nothing (no hipMalloc, hipFree or pinned host memory). n above 262144
+ * (2^18) returns CUIBM_ERR_ARGUMENT; precond 2 returns CUIBM_ERR_UNSUPPORTED (an AMG
+ * hierarchy is allocated). Every call on one GPU shares that array, so calls that
+ * overlap on one GPU race: serialise them. A is made beforehand by cuibm_csr_create,
+ * which does allocate. Return values as cuibm_bicgstab. */
+int cuibm_bicgstab_noalloc(cuibm_csr *A, double *x, const double *b, int precond, double rtol, int maxit,
+                           int *iters, double *resid);
 /* smoothed-aggregation hierarchy of A: level sizes and nonzeros; returns the level count */
 int cuibm_amg_levels(cuibm_csr *A, int *sizes, int *nnz, int cap);
 /* one V-cycle x = M^-1 b from a zero guess (host vectors) */
