@@ -1,0 +1,163 @@
+"""The benchmark configurations on the GPU against the oracle, at their real sizes.
+
+- 1024x1024 Re = 550 cylinder (Luo external), with the FP32-valued AMG hierarchy and with
+  the FP64 one, step by step against the FP64 oracle;
+- 336x336 Re = 40 cylinder with the modified Fadlun method, step by step;
+- both 336x336 cylinders over their full 2000 steps against the oracle's run of the same
+  case (fixtures written by tests/golden/make_cfg2_fixtures.py), with the drag against
+  Koumoutsakos & Leonard (1995);
+- the 64-case ensemble draw (edges 96-512, Re 40-200, 500 steps, FP32-valued hierarchy)
+  against each case run alone with the FP64 hierarchy, and its largest case against the
+  oracle.
+
+Bars: u within 1e-10 relative L2 per step; p within 1e-10 with its mean removed and the
+mean within 1e-8 (the Poisson operator is pinned by one doubled diagonal, so its constant
+mode is fixed only weakly; rounding moves it by up to 1e-9 while the velocity, which sees
+pressure differences only, agrees to 1e-12); drag and lift within 1e-8 in FP64; the mixed
+form within |dC_D| <= 1e-6 of the FP64 oracle.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cuibm_amd
+import pyoracle as po
+from cuibm_amd import ensemble as ens
+from helpers import case, pair, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TIGHT = {"velocityTol": 1e-12, "poissonTol": 1e-12}
+
+
+def _check_fields(g, o, k):
+    du = rel_l2(g.get("u"), o.get("u"))
+    pg, po_ = g.get("p"), o.get("p")
+    dp = rel_l2(pg - pg.mean(), po_ - po_.mean())
+    dm = abs(pg.mean() - po_.mean()) / np.sqrt(np.mean(po_ ** 2))
+    assert du < 1e-10, (k, "u", du)
+    assert dp < 1e-10, (k, "p", dp)
+    assert dm < 1e-8, (k, "p mean", dm)
+    return du, dp
+
+
+def test_cylinder_1024_mixed_and_fp64_match_oracle():
+    """The 1024x1024 Re = 550 cylinder over 10 steps of its impulsive start: the FP64 and
+    the FP32-valued hierarchy forms both against the FP64 oracle after every step."""
+    name = "cylinderRe550_1024_luo"
+    args = ["-velocityTol", "1e-12", "-poissonTol", "1e-12"]
+    g64 = cuibm_amd.Solver(case(name), *args, "-amgPrecision", "64", diagnostics=False)
+    g32 = cuibm_amd.Solver(case(name), *args, "-amgPrecision", "32", diagnostics=False)
+    g64.initialise()
+    g32.initialise()
+    o = po.Oracle(po.read_case(case(name), TIGHT))
+    for k in range(10):
+        g64.step_time()
+        g32.step_time()
+        o.step()
+        i64, i32, io = g64.info(), g32.info(), o.info()
+        d64 = _check_fields(g64, o, k)
+        d32 = _check_fields(g32, o, k)
+        cd = 2 * io["forceX"]
+        print(f"step {k + 1}: C_D oracle {cd:.12f} fp64 {2 * i64['forceX']:.12f} mixed {2 * i32['forceX']:.12f}"
+              f" | u {d64[0]:.1e} {d32[0]:.1e} p {d64[1]:.1e} {d32[1]:.1e} | Poisson its {i64['iter2']:.0f}"
+              f" {i32['iter2']:.0f} {io['iter2']:.0f}", flush=True)
+        for f in ("forceX", "forceY"):
+            assert abs(i64[f] - io[f]) <= 1e-8 * max(1.0, abs(io[f])), (k, f)
+        assert abs(2 * i32["forceX"] - cd) <= 1e-6 and abs(2 * i32["forceY"] - 2 * io["forceY"]) <= 1e-6, k
+    for s in (g64, g32):
+        s.close()
+
+
+def test_cylinder_336_fadlun_steps_match_oracle():
+    """The 336x336 cylinder with the modified Fadlun method (cut-cell LHS2, hybrid and ghost
+    LHS1 rows, masked projection), 10 steps, checked after every step."""
+    g, o = pair("cylinderRe40_fadlun", TIGHT)
+    for k in range(10):
+        g.step_time()
+        o.step()
+        _check_fields(g, o, k)
+        gi, oi = g.info(), o.info()
+        for f in ("forceX", "forceY"):
+            assert abs(gi[f] - oi[f]) <= 1e-8 * max(1.0, abs(oi[f])), (k, f, gi[f], oi[f])
+
+
+@pytest.mark.parametrize("method", ["luo", "fadlun"])
+def test_cylinder_336_full_run_matches_oracle_series(method):
+    """All 2000 steps of the 336x336 Re = 40 cylinder at its configured tolerances (1e-8)
+    against the oracle's run of the same case: drag and lift every 5 steps within 1e-8,
+    and every 200 steps the norms of u and of p (mean removed) within 1e-10 relative and u
+    at 16 nodes within 1e-10. The drag is also checked against Koumoutsakos & Leonard
+    (1995) at time * 0.5 (scripts/validation/validateCylinder.py:57): within 3% from t = 1."""
+    with open(os.path.join(GOLDEN, f"cfg2_{method}_series.json")) as f:
+        ref = json.load(f)
+    with open(os.path.join(GOLDEN, "literature.json")) as f:
+        kl = json.load(f)["kl95_re40"]
+    g = cuibm_amd.Solver(case(ref["case"]), diagnostics=False)
+    g.initialise()
+    idx = np.array(ref["probe_index"])
+    cd, cl = [], []
+    for k in range(1, ref["steps"] + 1):
+        g.step_time()
+        if k % ref["every"] == 0:
+            i = g.info()
+            cd.append(2 * i["forceX"])
+            cl.append(2 * i["forceY"])
+        if k % ref["snap"] == 0:
+            s = k // ref["snap"] - 1
+            u, p = g.get("u"), g.get("p")
+            du = abs(np.linalg.norm(u) - ref["u_norm"][s]) / ref["u_norm"][s]
+            dp = abs(np.linalg.norm(p - p.mean()) - ref["p_norm"][s]) / ref["p_norm"][s]
+            dprobe = np.max(np.abs(u[idx] - ref["u_probe"][s]))
+            print(f"{method} step {k}: C_D {cd[-1]:.12f} ({ref['cd'][len(cd) - 1]:.12f}) |u| {du:.1e} "
+                  f"|p| {dp:.1e} probes {dprobe:.1e}", flush=True)
+            assert du < 1e-10 and dp < 1e-10 and dprobe < 1e-10, k
+    cd, cl = np.array(cd), np.array(cl)
+    assert np.max(np.abs(cd - ref["cd"]) / np.abs(ref["cd"])) < 1e-8
+    assert np.max(np.abs(cl - ref["cl"])) < 1e-8
+    t = 0.01 * ref["every"] * np.arange(1, len(cd) + 1)
+    tk, ck = 0.5 * np.array(kl["t"]), np.array(kl["cd"])
+    m = (tk >= 1.0) & (tk <= t[-1])
+    ratio = np.interp(tk[m], t, cd) / ck[m]
+    print(method, "C_D / KL95 from t = 1:", np.round(ratio, 4), flush=True)
+    assert m.sum() >= 10 and np.all(np.abs(ratio - 1) < 0.03)
+    g.close()
+
+
+def test_ensemble_64_cases_against_cases_run_alone(tmp_path):
+    """The 64-case draw (seed 2024: edges 96-512, Re 40-200, h = D/40, 500 steps, Luo
+    external, FP32-valued Poisson hierarchy) run as one ensemble; every case's drag and lift
+    after its 500 steps within 1e-6 of the same case run alone with the FP64 hierarchy, and
+    the largest case against the oracle over its first 20 steps."""
+    steps = 500
+    paths, cases = ens.write_cases(str(tmp_path), n=64, seed=2024, steps=steps)
+    edges = [c[0] for c in cases]
+    print("edges", min(edges), max(edges), "cells", sum(e * e for e in edges), flush=True)
+    r = cuibm_amd.ensemble_run(paths, "-amgPrecision", "32", steps=steps, workers=16)
+    print(f"ensemble: {r['steps_wall_s']:.1f} s for the steps, "
+          f"{sum(e * e for e in edges) * steps / r['steps_wall_s'] / 1e6:.1f} M cell-updates/s", flush=True)
+    worst = 0.0
+    for i, p in enumerate(paths):
+        s = cuibm_amd.Solver(p, "-amgPrecision", "64", diagnostics=False)
+        s.initialise()
+        s.step_time(steps)
+        info = s.info()
+        s.close()
+        d = max(abs(2 * info["forceX"] - r["cd"][i]), abs(2 * info["forceY"] - r["cl"][i]))
+        worst = max(worst, d)
+        assert d <= 1e-6, (i, edges[i], d)
+    print("largest |dC_D|, |dC_L| against the FP64 runs:", worst, flush=True)
+    big = int(np.argmax(edges))
+    g = cuibm_amd.Solver(paths[big], "-amgPrecision", "64", diagnostics=False)
+    g.initialise()
+    o = po.Oracle(po.read_case(paths[big]))
+    g.step_time(20)
+    o.step(20)
+    gi, oi = g.info(), o.info()
+    print("largest case", edges[big], "C_D gpu", 2 * gi["forceX"], "oracle", 2 * oi["forceX"], flush=True)
+    assert rel_l2(g.get("u"), o.get("u")) < 1e-10
+    assert abs(gi["forceX"] - oi["forceX"]) <= 1e-8 * max(1.0, abs(oi["forceX"]))
+    g.close()
