@@ -6,10 +6,20 @@ Workload: impulsively started circular cylinder at Re=550 (case
 h=0.005 on [-0.6,0.6]^2 stretched by 1.008178 to 1024x1024 cells, Luo external
 ghost-node IBM with 628 body points, dt=0.0025. Velocity solve: BiCGSTAB + Jacobi,
 rtol 1e-9. Poisson solve: FP64 BiCGSTAB, rtol 1e-9, preconditioned by the
-smoothed-aggregation V-cycle (2+2 Jacobi sweeps, omega 0.8) whose matrix values are
-stored in FP32 while its vectors and arithmetic stay FP64 (the headline "mixed"
-form); the same steps are also run with an FP64 hierarchy and the drag coefficients
-of the two forms are compared.
+smoothed-aggregation V-cycle (2+2 Jacobi sweeps, omega 0.8) run in FP32 (matrix values,
+level vectors and smoother arithmetic), with the operator's pinned constant mode removed
+and restored in FP64 around it (the headline "mixed" form); the outer residual, the
+Krylov dot products and the projection stay FP64. The same steps are also run with an
+FP64 hierarchy and with FP64 CSR operators everywhere, and the drag coefficients of the
+forms are compared.
+
+The discrete adjoint (``adjoint``): the gradient of the mean drag over steps 200..400
+of a 400-step run of the rotating cylinder at Re=100 (case
+``cuibm_amd/cases/rotatingCylinderRe100_luo``: 498x498 stretched grid with h=0.01 near
+the body, 256 body points with tangential velocity alpha (D/2) omega_ref, alpha=0.5,
+omega_ref=2, plus seeded N(0, 0.01^2) normal blowing, dt=0.01, solves to 1e-12) with
+respect to alpha and the 256 blowing values, checkpoints every 20 steps; seconds per
+gradient = the forward run plus the backward sweep.
 
 A step is one full ``stepTime`` (rhs1, velocity solve, rhs2, Poisson solve, IBM
 interpolation/blending, projection, control-volume force) on fields resident in
@@ -24,13 +34,15 @@ Prints one JSON line. ``roofline`` reports the dominant kernel of the step (the
 fused finest-level pre-smoothing + residual kernel of the Poisson V-cycle, one per
 V-cycle and two V-cycles per BiCGSTAB iteration), timed with a HIP start/stop event
 pair attached to each of its launches (hipExtLaunchKernel, which stamps the kernel's
-own begin and end) inside a few further live time steps after the timed window (the
-timed window itself carries no probes); ``kernels`` lists back-to-back timings of
-single kernels on the live inputs for comparison; ``traffic`` is the
-HBM byte count per launch of that kernel from hardware counters, read from the
-committed summary under profiles/ when present. ``cpu_baseline`` is the FP64 C++
-CPU restatement of the same step (the oracle) timed on this host over a short
-sample of the same workload.
+own begin and end) over a few live time steps of a second instance of the same form,
+after the same warm-up (the timed window itself carries no probes); ``poisson`` is the
+Poisson solve alone, timed likewise with events before and after each whole solve;
+``kernels`` lists back-to-back timings of single kernels on the live inputs for
+comparison; ``traffic`` is the HBM byte count per launch of that kernel from hardware
+counters, read from the committed summary under profiles/ when that summary was taken on
+a kernel with the same algorithmic bytes per launch (null otherwise). ``cpu_baseline``
+is the FP64 C++ CPU restatement of the same step (the oracle) timed on this host over a
+short sample of the same workload, with the host's CPU count and model.
 """
 import argparse
 import json
@@ -89,11 +101,24 @@ def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5
     out["step_bytes"] = s.bytes(0) - s.bytes(1) - s.bytes(2) + out["poisson_bytes_per_step"] + \
         out["velocity_bytes_per_step"]
     out["hbm_gbps_step"] = out["step_bytes"] / (out["ms_per_step"] * 1e-3) / 1e9
+    s.close()
     if kernel:
-        # roofline kernel: HIP events attached to each of its launches inside `probe_steps`
-        # further live time steps (after the timed window, which stays unprobed)
+        # probes on a second instance of the same form, over the same early steps (W + 1
+        # onwards), so that the timed window above carries no events: the roofline kernel
+        # (HIP events attached to each of its launches) and every whole Poisson solve
+        # (events before and after each solve, host waits inside it included)
+        s = cu.Solver(case, "-amgPrecision", str(amg_precision), *extra, diagnostics=False)
+        s.initialise()
+        if warmup > 0:
+            s.step_time(warmup)
         kms, kbytes = s.time_kernel("live:" + kernel, probe_steps)
         out["kernel"] = {"name": kernel, "ms": kms, "bytes": kbytes, "steps": probe_steps}
+        i0 = s.info()
+        pms, pbytes = s.time_kernel("live:poisson", probe_steps)
+        its = (s.info()["iter2_total"] - i0["iter2_total"]) / max(probe_steps, 1)
+        out["poisson"] = {"ms_per_solve": pms, "bytes_per_solve": pbytes, "steps": probe_steps,
+                          "iterations_per_solve": its, "gbps": pbytes / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
+        out["poisson"].update(copy_ceiling(s, s.launch_bytes(), pms, its))
         out["kernels"] = {}
         if reps > 0:  # back-to-back launches of single kernels on the live inputs
             for k in dict.fromkeys([kernel] + EXTRA_KERNELS):
@@ -103,14 +128,67 @@ def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5
     return out
 
 
-def read_profile(key):
-    """a figure of the roofline kernel from the committed counter / trace summary"""
+def copy_ceiling(s, launches, ms_per_solve, its):
+    """The rate a plain copy kernel reaches at each launch size (back-to-back launches of
+    copy:N), and the time one Poisson iteration would take if each of its launches ran at
+    the copy's rate for its size: the attainable floor for launches of these sizes, set by
+    the dependent-dispatch gap at the small ones and by HBM at the large ones."""
+    import numpy as np
+
+    curve = []
+    for k in range(10, 27):  # 2^10 .. 2^26 doubles per array: 16 KB .. 1 GB moved
+        ms, b = s.time_kernel(f"copy:{2 ** k}", 50)
+        curve.append((b, ms))
+    cb = np.log([c[0] for c in curve])
+    ct = np.log([c[1] for c in curve])
+    floor = min(c[1] for c in curve)
+
+    def t_of(b):  # ms of a copy moving b bytes (log-log interpolation; the smallest copy's time below)
+        return max(floor, float(np.exp(np.interp(np.log(max(b, 1.0)), cb, ct))))
+
+    ceil_ms = sum(t_of(b) for b in launches)
+    meas_ms = ms_per_solve / its if its > 0 else 0.0
+    return {
+        "launches_per_iteration": len(launches),
+        "bytes_per_iteration": float(sum(launches)),
+        "ms_per_iteration": meas_ms,
+        "copy_ceiling_ms_per_iteration": ceil_ms,
+        "frac_of_copy_ceiling": ceil_ms / meas_ms if meas_ms > 0 else None,
+        "copy_ceiling_gbps": sum(launches) / (ceil_ms * 1e-3) / 1e9,
+        "copy_curve": [{"bytes": b, "us": round(ms * 1e3, 3), "gbps": round(b / (ms * 1e-3) / 1e9, 1)}
+                       for b, ms in curve],
+    }
+
+
+def read_profile(key, bytes_per_launch=None):
+    """a figure of the roofline kernel from the committed counter / trace summary; None
+    when the summary was taken on a kernel whose algorithmic bytes per launch differ from
+    the live kernel's (a stale summary)"""
     try:
         with open(TRAFFIC_FILE) as f:
-            m = re.search(key + r"\s*[:=]\s*([0-9.eE+]+)", f.read())
-        return float(m.group(1)) if m else None
+            text = f.read()
     except OSError:
         return None
+    if bytes_per_launch is not None:
+        m = re.search(r"algorithmic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", text)
+        if not m or abs(float(m.group(1)) - bytes_per_launch) > 1e-6 * bytes_per_launch:
+            return None
+    m = re.search(key + r"\s*[:=]\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def host_cpu():
+    """logical CPUs and the model name of this host (what lscpu prints)"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
 
 
 def cpu_baseline(case_dir, steps):
@@ -126,7 +204,7 @@ def cpu_baseline(case_dir, steps):
     o.step(steps)
     dt = time.perf_counter() - t0
     o.close()
-    return {"value": steps / dt, "unit": "time steps/s", "cores": threads, "kind": "port",
+    return {"value": steps / dt, "unit": "time steps/s", "cores": threads, "kind": "port", "host": host_cpu(),
             "sample": f"{steps} time steps (after 1 warm-up step) of the same 1024x1024 Luo cylinder case with "
                       f"the FP64 C++ oracle (FP64 SA-AMG), OpenMP threads={threads}; "
                       f"{dt:.1f} s of CPU work; initialisation excluded"}
@@ -177,9 +255,40 @@ def published_runs(cu):
     return rows
 
 
+def adjoint_bench(cu):
+    """one objective + gradient of the rotating cylinder (cuibm_amd.adjoint)"""
+    import numpy as np
+    from cuibm_amd.adjoint import RotatingCylinder
+
+    p = RotatingCylinder(os.path.join(cu.CASES, "rotatingCylinderRe100_luo"), nsteps=400, n0=200, n1=400, every=20)
+    blow = np.random.default_rng(2024).normal(0.0, 0.01, p.npts)
+    t0 = time.perf_counter()
+    J, ga, gb = p.evaluate(0.5, blow)
+    wall = time.perf_counter() - t0
+    r = p.last
+    log(f"[bench] adjoint: {wall:.2f} s per gradient (forward {r['forward_s']:.2f} s, backward {r['backward_s']:.2f} s)")
+    return {
+        "s_per_grad": round(wall, 3),
+        "forward_s": round(r["forward_s"], 3),
+        "backward_s": round(r["backward_s"], 3),
+        "backward_setup_s": round(r["backward_setup_s"], 3),
+        "backward_recompute_s": round(r["backward_recompute_s"], 3),
+        "forward_poisson_its": r["forward_poisson_its"],
+        "adjoint_poisson_its": r["adjoint_poisson_its"],
+        "adjoint_velocity_its": r["adjoint_velocity_its"],
+        "J": J,
+        "dJ_dalpha": ga,
+        "dJ_dblow_norm": float(np.linalg.norm(gb)),
+        "workload": "rotatingCylinderRe100_luo: 498x498, 256 points, alpha=0.5 + seeded blowing, 400 steps of "
+                    "dt=0.01, J = mean C_D over steps 200..400, gradient w.r.t. alpha and 256 blowing values, "
+                    "checkpoints every 20 steps, FP64, solves to 1e-12",
+        "timing": "wall clock of one call (fresh solver set-up, forward run, backward sweep)",
+    }
+
+
 def ensemble_bench(cu, n, steps, workers, check):
     """n seeded cylinder cases of 96^2..512^2 cells and Re 40..200 (cuibm_amd.ensemble),
-    run concurrently with the FP32-valued hierarchy; every (n // check)-th case is then
+    run concurrently with the FP32 V-cycle; every (n // check)-th case is then
     run alone with the FP64 hierarchy and its drag coefficient compared"""
     import tempfile
     from cuibm_amd import ensemble as ens
@@ -211,7 +320,7 @@ def ensemble_bench(cu, n, steps, workers, check):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n} impulsively started cylinders, edge 96..512 cells (seeded), Re 40..200, "
                                f"h=D/40 on [-0.6,0.6] stretched to [-10,10], dt=0.01, Luo external, Poisson FP64 "
-                               f"BiCGSTAB rtol 1e-8 + SA-AMG with FP32 values, velocity BiCGSTAB+Jacobi rtol 1e-8",
+                               f"BiCGSTAB rtol 1e-8 + FP32 SA-AMG V-cycle, velocity BiCGSTAB+Jacobi rtol 1e-8",
                    "cases": n, "steps": steps, "cells": cells, "workers": workers},
         "steps_wall_s": round(r["steps_wall_s"], 3), "wall_s_with_setup": round(r["wall_s"], 3),
         "alone_fp64_cell_updates_per_s": round(alone_rate, 1),
@@ -228,9 +337,11 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=100,
                     help="back-to-back launches per kernel for the kernel table (0: skip the table)")
     ap.add_argument("--probe-steps", type=int, default=5,
-                    help="live time steps after the timed window over which the roofline kernel is timed")
+                    help="live time steps (after the warm-up, on a second instance) over which the roofline "
+                         "kernel and the Poisson solve are timed")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-hierarchy comparison run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint gradient timing")
     ap.add_argument("--kernels-only", action="store_true",
                     help="initialise the case, run two time steps and the copy calibration kernel "
                          "(for hardware-counter runs)")
@@ -277,7 +388,8 @@ def main():
         s.close()
         print(json.dumps(out), flush=True)
         return
-    log(f"[bench] mixed form (FP32 AMG matrix values): warmup {args.warmup}, steps {args.steps}")
+    log(f"[bench] host: {host_cpu()}")
+    log(f"[bench] mixed form (FP32 V-cycle, FP64 deflation and Krylov): warmup {args.warmup}, steps {args.steps}")
     mixed = run_form(cu, case, 32, args.warmup, args.steps, args.kernel, args.kernel_reps, args.probe_steps)
     log(f"[bench] mixed: {mixed['steps_per_s']:.2f} steps/s, Poisson its/step {mixed['poisson_iters_per_step']:.1f}")
     fp64 = None
@@ -290,6 +402,10 @@ def main():
         log("[bench] FP64 form with stored CSR operators at every level (no stencil forms)")
         csr = run_form(cu, case, 64, args.warmup, args.steps, None, 0, extra=("-operators", "csr"))
         log(f"[bench] fp64 csr: {csr['steps_per_s']:.2f} steps/s, Poisson its/step {csr['poisson_iters_per_step']:.1f}")
+    adj = None
+    if not args.no_adjoint:
+        log("[bench] discrete adjoint of the rotating cylinder (498x498, 400 steps)")
+        adj = adjoint_bench(cu)
     cpu = None
     if args.cpu_steps > 0:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
@@ -313,13 +429,28 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "cylinderRe550_1024_luo: Re=550 cylinder, 1024x1024 stretched MAC grid, Luo external "
-                        "IBM, dt=0.0025; Poisson FP64 BiCGSTAB rtol 1e-9 + SA-AMG V-cycle with FP32 matrix values "
-                        "(FP64 vectors and arithmetic); velocity FP64 BiCGSTAB+Jacobi rtol 1e-9",
+                        "IBM, dt=0.0025; Poisson FP64 BiCGSTAB rtol 1e-9 preconditioned by an FP32 SA-AMG V-cycle "
+                        "(values, vectors, smoothers) with FP64 removal of the pinned constant mode; velocity FP64 "
+                        "BiCGSTAB+Jacobi rtol 1e-9",
             "grid": "1024x1024",
             "time_step_range": [mixed["time_step"] - args.steps + 1, mixed["time_step"]],
         },
         "hbm_gbps_step": round(mixed["hbm_gbps_step"], 1),
-        "poisson_gbps": round(mixed["poisson_bytes_per_step"] / (mixed["ms_per_step"] * 1e-3) / 1e9, 1),
+        "poisson": {
+            "gbps": round(mixed["poisson"]["gbps"], 1),
+            "frac_of_peak": round(mixed["poisson"]["gbps"] / HBM_PEAK_GBS, 4),
+            "ms_per_solve": round(mixed["poisson"]["ms_per_solve"], 4),
+            "bytes_per_solve": mixed["poisson"]["bytes_per_solve"],
+            "iterations_per_solve": mixed["poisson"]["iterations_per_solve"],
+            "ms_per_iteration": round(mixed["poisson"]["ms_per_iteration"], 4),
+            "copy_ceiling_ms_per_iteration": round(mixed["poisson"]["copy_ceiling_ms_per_iteration"], 4),
+            "frac_of_copy_ceiling": mixed["poisson"]["frac_of_copy_ceiling"],
+            "copy_ceiling_gbps": round(mixed["poisson"]["copy_ceiling_gbps"], 1),
+            "launches_per_iteration": mixed["poisson"]["launches_per_iteration"],
+            "copy_curve": mixed["poisson"]["copy_curve"],
+            "timing": f"HIP events before and after each whole Poisson solve (host waits inside it included) "
+                      f"over {mixed['poisson']['steps']} live time steps after the warm-up, on a second instance",
+        },
         "poisson_iters_per_step": mixed["poisson_iters_per_step"],
         "velocity_iters_per_step": mixed["velocity_iters_per_step"],
         "cd": mixed["cd"],
@@ -331,15 +462,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_profile("traffic_bytes_per_launch"),
-            "rocprof_us_per_launch": read_profile("rocprof_us_per_launch"),
+            "traffic": read_profile("traffic_bytes_per_launch", k["bytes"]),
+            "rocprof_us_per_launch": read_profile("rocprof_us_per_launch", k["bytes"]),
             "bytes_per_launch": k["bytes"],
             "us_per_launch": round(k["ms"] * 1e3, 2),
             "timing": f"HIP start/stop events attached to each in-step launch (hipExtLaunchKernel: the kernel's own "
-                      f"begin and end) over {k['steps']} live time steps after the timed window",
+                      f"begin and end) over {k['steps']} live time steps after the warm-up, on a second instance",
         },
         "kernels": {n: {kk: round(vv, 2) for kk, vv in d.items()} for n, d in mixed["kernels"].items()},
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     if fp64:
         line["fp64"] = {
@@ -351,7 +483,11 @@ def main():
             "cl": fp64["cl"],
         }
         line["speedup_mixed_vs_fp64"] = round(mixed["steps_per_s"] / fp64["steps_per_s"], 4)
+        line["speedup_target"] = 1.7
         line["abs_dcd_mixed_vs_fp64"] = abs(mixed["cd"] - fp64["cd"])
+        # where the time goes: the Poisson solve's share of each form's step, and its bytes
+        line["poisson_bytes_per_step"] = {"mixed": mixed["poisson_bytes_per_step"], "fp64": fp64["poisson_bytes_per_step"]}
+        line["step_bytes"] = {"mixed": mixed["step_bytes"], "fp64": fp64["step_bytes"]}
     if csr:
         line["fp64_csr"] = {
             "steps_per_s": round(csr["steps_per_s"], 4),

@@ -105,6 +105,10 @@ def lib():
     L.cuibm_ensemble_run.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int,
                                      C.POINTER(C.c_longlong), _DP, _DP, _DP, _DP, C.c_char_p, C.c_int]
     L.cuibm_ensemble_run.restype = C.c_int
+    L.cuibm_launch_bytes.argtypes = [C.c_void_p, C.c_char_p, _DP, C.c_long]
+    L.cuibm_launch_bytes.restype = C.c_long
+    L.cuibm_set_body_velocity.argtypes = [C.c_void_p, _DP, _DP, C.c_int, C.c_double]
+    L.cuibm_adjoint_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP]
     _lib = L
     return L
 
@@ -251,6 +255,40 @@ class Solver:
         ms = C.c_double(0)
         _check(lib().cuibm_time_steps(self._h, n, C.byref(ms)))
         return ms.value
+
+    # discrete adjoint (LUO solver, static body)
+    def set_body_velocity(self, ub, vb, omega=0.0):
+        """per-point body velocities ub[k], vb[k] and the rotation rate of the solid nodes
+        (cuibm_set_body_velocity), after initialise()"""
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        vb = np.ascontiguousarray(vb, dtype=np.float64)
+        if ub.shape != vb.shape or ub.ndim != 1:
+            raise ValueError("ub and vb must be 1-D arrays of one length")
+        _check(lib().cuibm_set_body_velocity(self._h, _dp(ub), _dp(vb), ub.size, float(omega)))
+
+    def adjoint_run(self, nsteps, n0, n1, every, backward=True):
+        """J = mean of C_D over steps n0..n1 of an nsteps run from time step 0 and, with
+        backward, dJ/dub, dJ/dvb (per body point) and dJ/domega (cuibm_adjoint_run)"""
+        npts = self.case_array("body_X").size
+        J = C.c_double(0)
+        gu, gv = np.zeros(npts), np.zeros(npts)
+        gom = C.c_double(0)
+        st = np.zeros(8)
+        _check(lib().cuibm_adjoint_run(self._h, int(nsteps), int(n0), int(n1), int(every), int(bool(backward)),
+                                       C.byref(J), _dp(gu), _dp(gv), C.byref(gom), _dp(st)))
+        out = {"J": J.value, "forward_s": st[0], "forward_poisson_its": st[2], "forward_velocity_its": st[3]}
+        if backward:
+            out.update({"grad_ub": gu, "grad_vb": gv, "grad_omega": gom.value, "backward_s": st[1],
+                        "adjoint_poisson_its": st[4], "adjoint_velocity_its": st[5], "backward_setup_s": st[6],
+                        "backward_recompute_s": st[7]})
+        return out
+
+    def launch_bytes(self, what="poisson_iteration"):
+        """algorithmic bytes of each launch of one Poisson BiCGSTAB iteration"""
+        n = _check(lib().cuibm_launch_bytes(self._h, what.encode(), None, 0))
+        a = np.zeros(n)
+        _check(lib().cuibm_launch_bytes(self._h, what.encode(), _dp(a), n))
+        return a
 
     def time_kernel(self, kernel, reps=20):
         ms = C.c_double(0)

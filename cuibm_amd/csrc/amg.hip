@@ -7,15 +7,15 @@
 // recursion, prolongation-correction, npost sweeps; the coarsest level (<= 2500 unknowns) applies a dense inverse, one
 // wavefront per row.
 //
-// Mixed precision stores only the matrix values (A_l, P_l, R_l, D_l^-1 and the
-// coarse inverse) as float; the level vectors and every product, sum and update stay
-// in FP64. Rounding the matrices perturbs a linear operator, so the preconditioner
-// stays linear and the outer BiCGSTAB converges as with the FP64 hierarchy. Rounding
-// the level vectors instead makes the V-cycle slightly non-linear; on the pinned
-// Neumann Poisson operator this lets BiCGSTAB iteration counts grow by orders of
-// magnitude over a run and end in breakdown.
+// Three precisions, as template parameters V (matrix values: A_l, P_l, R_l, D_l^-1 and
+// the coarse inverse) and X (level vectors and the arithmetic of products, sums and
+// updates): FP64 (double, double); FP32 values with FP64 vectors (float, double), where
+// rounding only perturbs a linear operator; and the FP32 V-cycle (float, float), which
+// reads the FP64 Krylov vector and writes its result in FP64 but keeps every level
+// vector, smoother sweep and product in FP32 (the coarsest solve accumulates in FP64).
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "krylov.hpp"
 #include "stencil.hpp"
@@ -29,12 +29,12 @@ int grid_for(long n) {
     return (int)(b < 65536 ? b : 65536);
 }
 
-template <class V>
-__global__ void k_jacobi_first(int n, const V *__restrict__ dinv, const double *__restrict__ b,
-                               double *__restrict__ x, double omega, const double *__restrict__ stop) {
+template <class V, class X>
+__global__ void k_jacobi_first(int n, const V *__restrict__ dinv, const X *__restrict__ b, X *__restrict__ x, X omega,
+                               const double *__restrict__ stop) {
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first loads
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double v = omega * (double)dinv[i] * b[i];
+        const X v = omega * (X)dinv[i] * b[i];
         if (stopv != 0.0) return;
         x[i] = v;
     }
@@ -50,18 +50,18 @@ __global__ void k_jacobi_first(int n, const V *__restrict__ dinv, const double *
 // matrix and LPR, never on scheduling.
 enum { OP_SPMV = 0, OP_RESID, OP_JACOBI, OP_PROLONG };
 
-template <class V, int LPR, int OP>
+template <class V, class X, int LPR, int OP>
 __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                                            const V *__restrict__ v, const double *__restrict__ x,
-                                            const double *__restrict__ b, const V *__restrict__ dinv,
-                                            double *__restrict__ y, double omega, const double *__restrict__ stop) {
+                                            const V *__restrict__ v, const X *__restrict__ x, const X *__restrict__ b,
+                                            const V *__restrict__ dinv, X *__restrict__ y, X omega,
+                                            const double *__restrict__ stop) {
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
-        double s = 0;
+        X s = 0;
         const int e = rp[i + 1];
-        for (int k = rp[i] + lane; k < e; k += LPR) s += (double)v[k] * x[ci[k]];
+        for (int k = rp[i] + lane; k < e; k += LPR) s += (X)v[k] * x[ci[k]];
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
         if (lane == 0) {
             if (OP == OP_SPMV) y[i] = s;
             else if (OP == OP_RESID) y[i] = b[i] - s;
-            else if (OP == OP_JACOBI) y[i] = x[i] + omega * (double)dinv[i] * (b[i] - s);
+            else if (OP == OP_JACOBI) y[i] = x[i] + omega * (X)dinv[i] * (b[i] - s);
             else y[i] += s;
         }
     }
@@ -78,20 +78,19 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
 // Restricted residual y = R (b - A x) as one product with G = [R | -R A]: column c < nf
 // of G multiplies b[c], column nf + j multiplies x[j]. Replaces the residual and the
 // restriction launches; the residual vector itself is never formed.
-template <class V, int LPR>
+template <class V, class X, int LPR>
 __global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restrict__ rp, const int *__restrict__ ci,
-                                             const V *__restrict__ v, const double *__restrict__ x,
-                                             const double *__restrict__ b, double *__restrict__ y,
-                                             const double *__restrict__ stop) {
+                                             const V *__restrict__ v, const X *__restrict__ x, const X *__restrict__ b,
+                                             X *__restrict__ y, const double *__restrict__ stop) {
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
-        double s = 0;
+        X s = 0;
         const int e = rp[i + 1];
         for (int k = rp[i] + lane; k < e; k += LPR) {
             const int c = ci[k];
-            s += (double)v[k] * (c < nf ? b[c] : x[c - nf]);
+            s += (X)v[k] * (c < nf ? b[c] : x[c - nf]);
         }
         if (LPR > 1)
 #pragma unroll
@@ -101,19 +100,18 @@ __global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restric
     }
 }
 
-template <class V>
-void launch_rres(int lpr, const DCsr<V> &G, int nf, const double *x, const double *b, double *y, hipStream_t s,
-                 const double *stop) {
+template <class V, class X>
+void launch_rres(int lpr, const DCsr<V> &G, int nf, const X *x, const X *b, X *y, hipStream_t s, const double *stop) {
     const int n = G.n;
     if (n <= 0) return;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_rres<V, 1>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        case 4: hipLaunchKernelGGL((k_rres<V, 4>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        case 8: hipLaunchKernelGGL((k_rres<V, 8>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        case 16: hipLaunchKernelGGL((k_rres<V, 16>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        default: hipLaunchKernelGGL((k_rres<V, 32>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 1: hipLaunchKernelGGL((k_rres<V, X, 1>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 4: hipLaunchKernelGGL((k_rres<V, X, 4>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 8: hipLaunchKernelGGL((k_rres<V, X, 8>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 16: hipLaunchKernelGGL((k_rres<V, X, 16>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        default: hipLaunchKernelGGL((k_rres<V, X, 32>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -122,44 +120,44 @@ void launch_rres(int lpr, const DCsr<V> &G, int nf, const double *x, const doubl
 // with x1 = omega D^-1 b evaluated at the row's neighbours on the fly. (Fusing the
 // prolongation into the first post-sweep the same way was measured slower: every row
 // then gathers its neighbours' whole P rows.)
-template <class V, int LPR>
+template <class V, class X, int LPR>
 __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                              const V *__restrict__ v, const V *__restrict__ dinv,
-                                             const double *__restrict__ b, double *__restrict__ x2, double omega,
+                                             const X *__restrict__ b, X *__restrict__ x2, X omega,
                                              const double *__restrict__ stop) {
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
-        double s = 0;
+        X s = 0;
         const int e = rp[i + 1];
         for (int k = rp[i] + lane; k < e; k += LPR) {
             const int j = ci[k];
-            s += (double)v[k] * (omega * (double)dinv[j] * b[j]);
+            s += (X)v[k] * (omega * (X)dinv[j] * b[j]);
         }
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
         if (stopv != 0.0) return;
         if (lane == 0) {
-            const double wd = omega * (double)dinv[i];
+            const X wd = omega * (X)dinv[i];
             x2[i] = wd * b[i] + wd * (b[i] - s);
         }
     }
 }
 
-template <class V>
-void launch_pre2(int lpr, const DCsr<V> &A, const V *dinv, const double *b, double *x2, double omega, hipStream_t s,
+template <class V, class X>
+void launch_pre2(int lpr, const DCsr<V> &A, const V *dinv, const X *b, X *x2, X omega, hipStream_t s,
                  const double *stop) {
     const int n = A.n;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_pre2<V, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        case 4: hipLaunchKernelGGL((k_pre2<V, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        case 8: hipLaunchKernelGGL((k_pre2<V, 8>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        case 16: hipLaunchKernelGGL((k_pre2<V, 16>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        default: hipLaunchKernelGGL((k_pre2<V, 32>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 1: hipLaunchKernelGGL((k_pre2<V, X, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 4: hipLaunchKernelGGL((k_pre2<V, X, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 8: hipLaunchKernelGGL((k_pre2<V, X, 8>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 16: hipLaunchKernelGGL((k_pre2<V, X, 16>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        default: hipLaunchKernelGGL((k_pre2<V, X, 32>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -174,19 +172,19 @@ int lanes_for(const HCsr &A) {
     return 32;
 }
 
-template <class V, int OP>
-void launch_csr_op(int lpr, const DCsr<V> &A, const double *x, const double *b, const V *dinv, double *y, double omega,
-                   hipStream_t s, const double *stop = nullptr) {
+template <class V, class X, int OP>
+void launch_csr_op(int lpr, const DCsr<V> &A, const X *x, const X *b, const V *dinv, X *y, X omega, hipStream_t s,
+                   const double *stop = nullptr) {
     const int n = A.n;
     if (n <= 0) return;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_csr<V, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        case 4: hipLaunchKernelGGL((k_csr<V, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        case 8: hipLaunchKernelGGL((k_csr<V, 8, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        case 16: hipLaunchKernelGGL((k_csr<V, 16, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        default: hipLaunchKernelGGL((k_csr<V, 32, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 1: hipLaunchKernelGGL((k_csr<V, X, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 4: hipLaunchKernelGGL((k_csr<V, X, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 8: hipLaunchKernelGGL((k_csr<V, X, 8, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 16: hipLaunchKernelGGL((k_csr<V, X, 16, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        default: hipLaunchKernelGGL((k_csr<V, X, 32, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -209,9 +207,9 @@ __device__ __forceinline__ void load4<double>(const double *p, double (&v)[4]) {
     v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
 }
 
-template <class V>
-__global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restrict__ Ainv, const double *__restrict__ b,
-                                               double *__restrict__ x, const double *__restrict__ stop) {
+template <class V, class X>
+__global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restrict__ Ainv, const X *__restrict__ b,
+                                               X *__restrict__ x, const double *__restrict__ stop) {
     const double stopv = stop ? *stop : 0.0;
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -223,7 +221,7 @@ __global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restric
     for (int j = 4 * lane; j < n4; j += 256) {
         double av[4], bv[4];
         load4<V>(a + j, av);
-        load4<double>(b + j, bv);
+        load4<X>(b + j, bv);
         if (!tested) {
             if (stopv != 0.0) return;
             tested = true;
@@ -238,35 +236,41 @@ __global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restric
     double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if (lane == 0) x[row] = s;
+    if (lane == 0) x[row] = (X)s;
 }
 
 }  // namespace
 
-template <class V>
+template <class V, class X>
 struct AmgLevelDev {
     int n = 0;
     int lanesA = 1, lanesP = 1, lanesR = 1, lanesG = 1;  // lanes per row of each operator
     DCsr<V> A, P, R, G;
     DArray<V> dinv;
-    DArray<double> b, x, xt, r;
+    DArray<X> b, x, xt, r;
 };
 
-template <class V>
+template <class V, class X>
 struct AmgDev {
-    std::vector<AmgLevelDev<V>> lv;
+    std::vector<AmgLevelDev<V, X>> lv;
     DArray<V> coarse_inv;
+    DArray<float> fine_dinv32;  // FP32 D^-1 of the stencil finest level (FP32 V-cycle)
+    // FP64 deflation of the pinned constant mode (FP32 V-cycle on a stencil finest level):
+    // partial sums, c, and 1 / sum over the pins of area * (A 1)
+    DArray<double> defl_part, defl_c;
+    double inv_wa = 0;
     int nc = 0, ld = 0;
     AmgParams prm;
     bool use_fine = false;  // finest level in stencil form (fused kernels)
     P5Dev fine{};
     KernelProbe *probe = nullptr;  // events around the finest-level pre-smoothing launches
     const double *stop = nullptr;  // the Krylov solver's stop flag (every kernel checks it)
+    static constexpr bool kVec64 = std::is_same<X, double>::value;
 
     void build(const std::vector<HostAmgLevel> &h, hipStream_t s) {
         lv.resize(h.size());
         for (size_t l = 0; l < h.size(); l++) {
-            AmgLevelDev<V> &L = lv[l];
+            AmgLevelDev<V, X> &L = lv[l];
             L.n = h[l].A.n;
             const bool stencil_level = (l == 0 && use_fine);
             if (!stencil_level) upload_csr(h[l].A, L.A, s);
@@ -291,12 +295,19 @@ struct AmgDev {
                 }
                 L.dinv.upload(d, s);
             }
-            if (l > 0) {  // level 0 works on the caller's vectors
+            // level 0 works on the caller's FP64 vectors, except a CSR finest level of
+            // the FP32 V-cycle, which converts them into and out of its own
+            if (l > 0 || (!kVec64 && !stencil_level)) {
                 L.b.alloc(L.n);
                 L.x.alloc(L.n);
             }
             L.xt.alloc(L.n);
             L.r.alloc(L.n);
+        }
+        if (use_fine && !kVec64) {
+            fine_dinv32.alloc(lv[0].n);
+            launch_to_float(fine_dinv32.p(), fine.dinv, lv[0].n, s);
+            fine.dinv32 = fine_dinv32.p();
         }
         nc = h.back().A.n;
         if (nc > 6144) throw Error("AMG coarsening stalled at " + std::to_string(nc) + " unknowns", ERR_GENERIC);
@@ -309,100 +320,161 @@ struct AmgDev {
         CUIBM_HIP(hipStreamSynchronize(s));
     }
 
-    // V-cycle on level l with right-hand side b, result into x (both level-sized)
-    void vcycle(size_t l, const double *b, double *x, hipStream_t s) {
-        AmgLevelDev<V> &L = lv[l];
+    // the preconditioner: one V-cycle from x = 0 on the FP64 Krylov vectors b and x
+    void apply(const double *b, double *x, hipStream_t s) {
+        const X w = (X)prm.omega;
+        if (use_fine && prm.npre == 2 && prm.npost == 2 && lv.size() > 1) {
+            // stencil finest level: x2 -> xt and r -> r, restriction, coarse cycle,
+            // prolongation into xt, both post-sweeps straight into x
+            AmgLevelDev<V, X> &L = lv[0], &C = lv[1];
+            if (fine.defl) launch_p5_deflate(fine, b, defl_part.p(), defl_c.p(), inv_wa, s);
+            launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s, probe);
+            launch_csr_op<V, X, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), X(0), s, stop);
+            vcycle(1, C.b.p(), C.x.p(), s);
+            launch_csr_op<V, X, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), X(0), s, stop);
+            launch_p5_sweep2(fine, w, L.xt.p(), b, x, s);
+            return;
+        }
+        if constexpr (kVec64) {
+            vcycle(0, b, x, s);
+        } else {
+            AmgLevelDev<V, X> &L = lv[0];
+            launch_to_float(L.b.p(), b, L.n, s);
+            vcycle(0, L.b.p(), L.x.p(), s);
+            launch_to_double(x, L.x.p(), L.n, s);
+        }
+    }
+
+    // V-cycle on level l with right-hand side b, result into x (both level vectors)
+    void vcycle(size_t l, const X *b, X *x, hipStream_t s) {
+        AmgLevelDev<V, X> &L = lv[l];
         const int n = L.n;
-        const double w = prm.omega;
+        const X w = (X)prm.omega;
         if (l + 1 == lv.size()) {
             if ((reinterpret_cast<uintptr_t>(b) & 15) != 0)  // the kernel reads b four values at a time
                 throw Error("coarsest AMG right-hand side is not 16-byte aligned", ERR_ARGUMENT);
-            hipLaunchKernelGGL(k_dense<V>, dim3((n + 3) / 4), dim3(256), 0, s, n, ld, coarse_inv.p(), b, x,
+            hipLaunchKernelGGL((k_dense<V, X>), dim3((n + 3) / 4), dim3(256), 0, s, n, ld, coarse_inv.p(), b, x,
                                stop);
             CUIBM_LAUNCH_CHECK();
-            return;
-        }
-        if (l == 0 && use_fine && prm.npre == 2 && prm.npost == 2) {
-            // stencil finest level: x2 -> xt and r -> r, restriction, coarse cycle,
-            // prolongation into xt, both post-sweeps straight into x
-            launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s, probe);
-            AmgLevelDev<V> &C = lv[1];
-            launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
-            vcycle(1, C.b.p(), C.x.p(), s);
-            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s, stop);
-            launch_p5_sweep2(fine, w, L.xt.p(), b, x, s);
             return;
         }
         if (prm.npre == 2 && prm.npost == 2) {
             // x2 -> xt, r -> r, restriction, coarse cycle, prolongation into xt,
             // post-sweeps xt -> r -> x
-            AmgLevelDev<V> &C = lv[l + 1];
-            launch_pre2<V>(L.lanesA, L.A, L.dinv.p(), b, L.xt.p(), w, s, stop);
+            AmgLevelDev<V, X> &C = lv[l + 1];
+            launch_pre2<V, X>(L.lanesA, L.A, L.dinv.p(), b, L.xt.p(), w, s, stop);
             if (L.G.n > 0) {
-                launch_rres<V>(L.lanesG, L.G, n, L.xt.p(), b, C.b.p(), s, stop);  // R (b - A x2)
+                launch_rres<V, X>(L.lanesG, L.G, n, L.xt.p(), b, C.b.p(), s, stop);  // R (b - A x2)
             } else {
-                launch_csr_op<V, OP_RESID>(L.lanesA, L.A, L.xt.p(), b, nullptr, L.r.p(), 0.0, s, stop);
-                launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
+                launch_csr_op<V, X, OP_RESID>(L.lanesA, L.A, L.xt.p(), b, nullptr, L.r.p(), X(0), s, stop);
+                launch_csr_op<V, X, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), X(0), s, stop);
             }
             vcycle(l + 1, C.b.p(), C.x.p(), s);
-            launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), 0.0, s, stop);
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), b, L.dinv.p(), L.r.p(), w, s, stop);
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, L.r.p(), b, L.dinv.p(), x, w, s, stop);
+            launch_csr_op<V, X, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), X(0), s, stop);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), b, L.dinv.p(), L.r.p(), w, s, stop);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.r.p(), b, L.dinv.p(), x, w, s, stop);
             return;
         }
         const int g = grid_for(n);
         // ping-pong so that the last sweep lands in x
         const int swaps = (prm.npre - 1) + prm.npost;
-        double *cur = (swaps % 2 == 0) ? x : L.xt.p();
-        double *oth = (cur == x) ? L.xt.p() : x;
-        hipLaunchKernelGGL(k_jacobi_first<V>, dim3(g), dim3(BS), 0, s, n, L.dinv.p(), b, cur, w, stop);
+        X *cur = (swaps % 2 == 0) ? x : L.xt.p();
+        X *oth = (cur == x) ? L.xt.p() : x;
+        hipLaunchKernelGGL((k_jacobi_first<V, X>), dim3(g), dim3(BS), 0, s, n, L.dinv.p(), b, cur, w, stop);
         CUIBM_LAUNCH_CHECK();
         for (int k = 1; k < prm.npre; k++) {
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s, stop);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s, stop);
             std::swap(cur, oth);
         }
-        launch_csr_op<V, OP_RESID>(L.lanesA, L.A, cur, b, nullptr, L.r.p(), 0.0, s, stop);
-        AmgLevelDev<V> &C = lv[l + 1];
-        launch_csr_op<V, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), 0.0, s, stop);
+        launch_csr_op<V, X, OP_RESID>(L.lanesA, L.A, cur, b, nullptr, L.r.p(), X(0), s, stop);
+        AmgLevelDev<V, X> &C = lv[l + 1];
+        launch_csr_op<V, X, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), X(0), s, stop);
         vcycle(l + 1, C.b.p(), C.x.p(), s);
-        launch_csr_op<V, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, cur, 0.0, s, stop);
+        launch_csr_op<V, X, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, cur, X(0), s, stop);
         for (int k = 0; k < prm.npost; k++) {
-            launch_csr_op<V, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s, stop);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, cur, b, L.dinv.p(), oth, w, s, stop);
             std::swap(cur, oth);
         }
     }
 };
 
 struct AmgPrecond::Impl {
-    AmgDev<double> d64;
-    AmgDev<float> d32;
+    AmgDev<double, double> d64;
+    AmgDev<float, double> d3264;
+    AmgDev<float, float> d32;
     std::vector<HostAmgLevel> host;
     KernelProbe probe;
+    template <class F>
+    void each(F f) {
+        f(d64);
+        f(d3264);
+        f(d32);
+    }
 };
 
 KernelProbe &AmgPrecond::pre_probe() { return im_->probe; }
 
 AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine)
     : im_(new Impl), prm_(p) {
+    if (p.precision != 64 && p.precision != 32 && p.precision != 3264)
+        throw Error("AMG precision must be 64, 32 or 3264", ERR_ARGUMENT);
     im_->host = amg_build(A, p.coarse_max, p.max_levels, p.scale0, p.fused_restriction);
     fine_ = fine != nullptr && im_->host.size() > 1 && p.npre == 2 && p.npost == 2;
     if (fine_) {
         if (fine->nx * fine->ny != A.n) throw Error("stencil and matrix sizes differ", ERR_ARGUMENT);
         fine_dev_ = *fine;
     }
-    im_->d32.probe = &im_->probe;
-    im_->d64.probe = &im_->probe;
-    if (p.precision == 32) {
-        im_->d32.prm = p;
-        im_->d32.use_fine = fine_;
-        if (fine_) im_->d32.fine = *fine;
-        im_->d32.build(im_->host, s);
-    } else {
-        im_->d64.prm = p;
-        im_->d64.use_fine = fine_;
-        if (fine_) im_->d64.fine = *fine;
-        im_->d64.build(im_->host, s);
-    }
+    auto setup = [&](auto &d) {
+        d.probe = &im_->probe;
+        d.prm = p;
+        d.use_fine = fine_;
+        if (fine_) d.fine = *fine;
+        d.build(im_->host, s);
+        if (fine_) fine_dev_.dinv32 = d.fine.dinv32;
+    };
+    if (p.precision == 32 && fine_) {
+        // The FP32 V-cycle loses the weakly pinned constant mode of the Poisson operator
+        // (A = A0 + E with A0 1 = 0 and E nonzero only at the pinned cells), and BiCGSTAB
+        // then needs several times the iterations. The cell areas w make W A0 symmetric, so
+        // w^T A0 = 0 and w^T A = (w o a)^T with a = A 1 nonzero only at the pins. Writing
+        // r = r' + c a with c = w^T r / w^T a gives A^-1 r = A^-1 r' + c 1, where r' has no
+        // component along the constant mode; the V-cycle sees r' and c 1 is added in FP64.
+        auto &d = im_->d32;
+        const HCsr &H = A;
+        auto rowsum = [&](int i) {
+            double acc = 0;
+            for (int k = H.rp[i]; k < H.rp[i + 1]; k++) acc += H.v[k];
+            return acc;
+        };
+        const int nx = fine->nx, cc = fine->centre;
+        auto area = [&](int ip) {
+            double dx = 0, dy = 0;
+            CUIBM_HIP(hipMemcpy(&dx, fine->dx + ip % nx, sizeof(double), hipMemcpyDeviceToHost));
+            CUIBM_HIP(hipMemcpy(&dy, fine->dy + ip / nx, sizeof(double), hipMemcpyDeviceToHost));
+            return dx * dy;
+        };
+        if (!fine->dx || !fine->dy) throw Error("deflation needs the cell widths of the stencil operator", ERR_ARGUMENT);
+        CUIBM_HIP(hipStreamSynchronize(s));
+        const double a0 = rowsum(0), ac = cc > 0 ? rowsum(cc) : 0.0;
+        const double wa = area(0) * a0 + (cc > 0 ? area(cc) * ac : 0.0);
+        if (!(wa > 0)) throw Error("the Poisson operator has no pinned cell to deflate", ERR_ARGUMENT);
+        d.defl_part.alloc(RED_GRID);
+        d.defl_c.alloc(1);
+        launch_fill(d.defl_c.p(), 0.0, 1, s);
+        d.inv_wa = 1.0 / wa;
+        P5Dev f = *fine;
+        f.defl = d.defl_c.p();
+        f.defl_a0 = a0;
+        f.defl_ac = ac;
+        d.probe = &im_->probe;
+        d.prm = p;
+        d.use_fine = true;
+        d.fine = f;
+        d.build(im_->host, s);
+        fine_dev_ = d.fine;
+    } else if (p.precision == 32) setup(im_->d32);
+    else if (p.precision == 3264) setup(im_->d3264);
+    else setup(im_->d64);
     // keep only the sizes of the host hierarchy
     for (auto &L : im_->host) {
         L.A.v.clear(); L.A.v.shrink_to_fit();
@@ -416,31 +488,38 @@ AmgPrecond::~AmgPrecond() = default;
 
 void AmgPrecond::set_stop(const double *stop) {
     stop_ = stop;
-    im_->d64.stop = stop;
-    im_->d32.stop = stop;
-    im_->d64.fine.stop = stop;
-    im_->d32.fine.stop = stop;
+    im_->each([&](auto &d) {
+        d.stop = stop;
+        d.fine.stop = stop;
+    });
 }
 
 void AmgPrecond::apply(const double *r, double *z, hipStream_t s) {
-    if (prm_.precision == 32) im_->d32.vcycle(0, r, z, s);
-    else im_->d64.vcycle(0, r, z, s);
+    if (prm_.precision == 32) im_->d32.apply(r, z, s);
+    else if (prm_.precision == 3264) im_->d3264.apply(r, z, s);
+    else im_->d64.apply(r, z, s);
 }
 
 void AmgPrecond::level0_sweep(const double *b, const double *x, double *xn, hipStream_t s) {
     if (fine_) {
         launch_p5_sweep(fine_dev_, prm_.omega, x, b, xn, s);
-    } else if (prm_.precision == 32) {
-        auto &L = im_->d32.lv[0];
-        launch_csr_op<float, OP_JACOBI>(L.lanesA, L.A, x, b, L.dinv.p(), xn, prm_.omega, s);
-    } else {
+    } else if (prm_.precision == 64) {
         auto &L = im_->d64.lv[0];
-        launch_csr_op<double, OP_JACOBI>(L.lanesA, L.A, x, b, L.dinv.p(), xn, prm_.omega, s);
+        launch_csr_op<double, double, OP_JACOBI>(L.lanesA, L.A, x, b, L.dinv.p(), xn, prm_.omega, s);
+    } else if (prm_.precision == 3264) {  // the FP32-valued sweep on FP64 vectors
+        auto &L = im_->d3264.lv[0];
+        launch_csr_op<float, double, OP_JACOBI>(L.lanesA, L.A, x, b, L.dinv.p(), xn, prm_.omega, s);
+    } else {
+        throw Error("level-0 sweep timing of the FP32 V-cycle needs its stencil finest level", ERR_UNSUPPORTED);
     }
 }
 
+// bytes of a value of the matrices (sv) and of a level vector (sx)
+static double value_bytes(int precision) { return precision == 64 ? 8.0 : 4.0; }
+static double vector_bytes(int precision) { return precision == 32 ? 4.0 : 8.0; }
+
 double AmgPrecond::level0_sweep_bytes() const {
-    const double sv = prm_.precision == 32 ? 4.0 : 8.0;
+    const double sv = value_bytes(prm_.precision);
     const auto &A = im_->host[0].A;
     const double n = A.n, nnz = (double)A.ci.size();
     if (fine_) return n * (8 + 8 + 8) + 32.0 * (fine_dev_.nx + fine_dev_.ny);  // x, b, xn, coefficient arrays
@@ -449,30 +528,43 @@ double AmgPrecond::level0_sweep_bytes() const {
 
 void AmgPrecond::level0_pre(const double *b, double *x2, double *r, hipStream_t s) {
     if (!fine_) throw Error("the finest AMG level is not in stencil form", ERR_ARGUMENT);
-    launch_p5_pre(fine_dev_, prm_.omega, b, x2, r, s);
+    if (prm_.precision == 32)  // FP32 level vectors in the caller's (larger) buffers
+        launch_p5_pre(fine_dev_, (float)prm_.omega, b, reinterpret_cast<float *>(x2), reinterpret_cast<float *>(r),
+                      s);
+    else
+        launch_p5_pre(fine_dev_, prm_.omega, b, x2, r, s);
 }
 
 void AmgPrecond::level0_post(double *x2, const double *b, double *x, hipStream_t s) {
     if (!fine_) throw Error("the finest AMG level is not in stencil form", ERR_ARGUMENT);
     if (prm_.precision == 32) {
         auto &L = im_->d32.lv[0];
-        launch_csr_op<float, OP_PROLONG>(L.lanesP, L.P, im_->d32.lv[1].x.p(), nullptr, nullptr, x2, 0.0, s);
+        float *x2f = reinterpret_cast<float *>(x2);
+        launch_csr_op<float, float, OP_PROLONG>(L.lanesP, L.P, im_->d32.lv[1].x.p(), nullptr, nullptr, x2f, 0.0f, s);
+        launch_p5_sweep2(fine_dev_, (float)prm_.omega, x2f, b, x, s);
+    } else if (prm_.precision == 3264) {
+        auto &L = im_->d3264.lv[0];
+        launch_csr_op<float, double, OP_PROLONG>(L.lanesP, L.P, im_->d3264.lv[1].x.p(), nullptr, nullptr, x2, 0.0, s);
+        launch_p5_sweep2(fine_dev_, prm_.omega, x2, b, x, s);
     } else {
         auto &L = im_->d64.lv[0];
-        launch_csr_op<double, OP_PROLONG>(L.lanesP, L.P, im_->d64.lv[1].x.p(), nullptr, nullptr, x2, 0.0, s);
+        launch_csr_op<double, double, OP_PROLONG>(L.lanesP, L.P, im_->d64.lv[1].x.p(), nullptr, nullptr, x2, 0.0, s);
+        launch_p5_sweep2(fine_dev_, prm_.omega, x2, b, x, s);
     }
-    launch_p5_sweep2(fine_dev_, prm_.omega, x2, b, x, s);
 }
 
 double AmgPrecond::level0_pre_bytes() const {
-    return 4.0 * 8 * im_->host[0].A.n + 32.0 * (fine_dev_.nx + fine_dev_.ny);  // b, D^-1 in; x2, r out
+    const double sx = vector_bytes(prm_.precision);
+    // b (the FP64 Krylov vector) and D^-1 in; x2, r out
+    return (8.0 + 3 * sx) * im_->host[0].A.n + 32.0 * (fine_dev_.nx + fine_dev_.ny);
 }
 
 double AmgPrecond::level0_post_bytes() const {
-    const double sv = prm_.precision == 32 ? 4.0 : 8.0;
+    const double sv = value_bytes(prm_.precision), sx = vector_bytes(prm_.precision);
     const double n = im_->host[0].A.n, nc = im_->host[1].A.n, nnzP = (double)im_->host[0].P.ci.size();
-    const double prolong = nnzP * (sv + 4) + 4 * (n + 1) + nc * 8 + 2 * n * 8;            // P, xc; x2 read-modify-write
-    const double sweeps = 4.0 * 8 * n + 32.0 * (fine_dev_.nx + fine_dev_.ny);            // x, b, D^-1 in; x out
+    const double prolong = nnzP * (sv + 4) + 4 * (n + 1) + nc * sx + 2 * n * sx;  // P, xc; x2 read-modify-write
+    // x2 and D^-1 in as level vectors, b in and the result out as FP64 Krylov vectors
+    const double sweeps = (2 * sx + 16.0) * n + 32.0 * (fine_dev_.nx + fine_dev_.ny);
     return prolong + sweeps;
 }
 
@@ -487,38 +579,59 @@ AmgHierarchyInfo AmgPrecond::info() const {
     return I;
 }
 
-double AmgPrecond::bytes_per_apply() const {
-    // distinct arrays per kernel; matrix values and D^-1 take sv bytes, vectors 8:
-    // CSR product = nnz*(sv+4) + 4(n+1) + 8 ncol (x) + 8 n (y)
-    const double sv = prm_.precision == 32 ? 4.0 : 8.0;
-    double bytes = 0;
+std::vector<double> AmgPrecond::launch_bytes() const {
+    // distinct arrays per kernel; matrix values and D^-1 take sv bytes, level vectors sx:
+    // CSR product = nnz*(sv+4) + 4(n+1) + sx ncol (x) + sx n (y)
+    const double sv = value_bytes(prm_.precision), sx = vector_bytes(prm_.precision);
+    std::vector<double> L;
     const auto &H = im_->host;
-    auto spmv = [&](double n, double ncol, double nnz) { return nnz * (sv + 4) + 4 * (n + 1) + ncol * 8 + n * 8; };
+    auto spmv = [&](double n, double ncol, double nnz) { return nnz * (sv + 4) + 4 * (n + 1) + ncol * sx + n * sx; };
     for (size_t l = 0; l + 1 < H.size(); l++) {
         const double n = H[l].A.n, nnz = (double)H[l].A.ci.size();
         const double nc = H[l + 1].A.n, nnzP = (double)H[l].P.ci.size();
         if (l == 0 && fine_) {
-            bytes += level0_pre_bytes() + spmv(nc, n, nnzP) + level0_post_bytes();  // pre, restriction, post
+            if (fine_dev_.defl) {  // weighted sum of b, its one-block finish
+                L.push_back(8.0 * n + 8.0 * (fine_dev_.nx + fine_dev_.ny));
+                L.push_back(8.0 * RED_GRID);
+            }
+            L.push_back(level0_pre_bytes());
+            L.push_back(spmv(nc, n, nnzP));                                          // restriction
+            L.push_back(nnzP * (sv + 4) + 4 * (n + 1) + nc * sx + 2 * n * sx);        // prolong-add
+            L.push_back((2 * sx + 16.0) * n + 32.0 * (fine_dev_.nx + fine_dev_.ny));  // both post-sweeps
             continue;
+        }
+        if (l == 0 && prm_.precision == 32) {  // FP64 <-> FP32 conversions of b and x
+            L.push_back(n * (8 + sx));
+            L.push_back(n * (8 + sx));
         }
         if (prm_.npre == 2 && prm_.npost == 2) {  // the fused (2,2) sequence of vcycle()
             const double nnzG = (double)H[l].G.ci.size();
-            bytes += spmv(n, n, nnz) + n * sv + n * 8;                    // both pre-sweeps (+ dinv, b)
-            if (nnzG > 0) bytes += nnzG * (sv + 4) + 4 * (nc + 1) + 2 * n * 8 + nc * 8;  // R (b - A x2) via G
-            else bytes += spmv(n, n, nnz) + n * 8 + spmv(nc, n, nnzP);    // residual (+ b), restriction
-            bytes += spmv(n, nc, nnzP) + n * 8;                           // prolong-add
-            bytes += 2 * (spmv(n, n, nnz) + n * sv + n * 8);              // post-sweeps (+ dinv, b)
+            L.push_back(spmv(n, n, nnz) + n * sv + n * sx);  // both pre-sweeps (+ dinv, b)
+            if (nnzG > 0) {
+                L.push_back(nnzG * (sv + 4) + 4 * (nc + 1) + 2 * n * sx + nc * sx);  // R (b - A x2) via G
+            } else {
+                L.push_back(spmv(n, n, nnz) + n * sx);  // residual (+ b)
+                L.push_back(spmv(nc, n, nnzP));         // restriction
+            }
+            L.push_back(spmv(n, nc, nnzP) + n * sx);  // prolong-add
+            for (int k = 0; k < 2; k++) L.push_back(spmv(n, n, nnz) + n * sv + n * sx);  // post-sweeps (+ dinv, b)
             continue;
         }
-        bytes += n * sv + 2 * n * 8;                                   // first sweep: dinv, b, x
+        L.push_back(n * sv + 2 * n * sx);  // first sweep: dinv, b, x
         const int sweeps = prm_.npre - 1 + prm_.npost;
-        bytes += sweeps * (spmv(n, n, nnz) + n * sv + n * 8);          // + dinv, b
-        bytes += spmv(n, n, nnz) + n * 8;                              // residual (+ b)
-        bytes += spmv(nc, n, nnzP);                                    // restriction
-        bytes += spmv(n, nc, nnzP) + n * 8;                            // prolong-add (read-modify-write x)
+        for (int k = 0; k < sweeps; k++) L.push_back(spmv(n, n, nnz) + n * sv + n * sx);  // + dinv, b
+        L.push_back(spmv(n, n, nnz) + n * sx);  // residual (+ b)
+        L.push_back(spmv(nc, n, nnzP));         // restriction
+        L.push_back(spmv(n, nc, nnzP) + n * sx);  // prolong-add (read-modify-write x)
     }
     const double nc = H.back().A.n;
-    bytes += nc * nc * sv + 2 * nc * 8;
+    L.push_back(nc * nc * sv + 2 * nc * sx);
+    return L;
+}
+
+double AmgPrecond::bytes_per_apply() const {
+    double bytes = 0;
+    for (double b : launch_bytes()) bytes += b;
     return bytes;
 }
 

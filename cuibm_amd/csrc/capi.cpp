@@ -24,10 +24,9 @@ struct cuibm_csr {
     DCsr<double> A;
     HCsr h;
     hipStream_t stream = nullptr;
-    std::unique_ptr<AmgPrecond> amg[2];
+    std::unique_ptr<AmgPrecond> amg[3];  // FP64, FP32 V-cycle, FP32 values
     ~cuibm_csr() {
-        amg[0].reset();
-        amg[1].reset();
+        for (auto &a : amg) a.reset();
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -299,6 +298,17 @@ int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms,
         return s->s->timeKernel(kernel ? kernel : "", reps, ms, bytes); });
 }
 
+long cuibm_launch_bytes(cuibm_solver *s, const char *what, double *out, long cap) {
+    return guardl([&] { need(s);
+        const std::string w = what ? what : "";
+        if (w != "poisson_iteration") throw Error("unknown launch list '" + w + "'", ERR_ARGUMENT);
+        const std::vector<double> L = s->s->poissonIterationLaunches();
+        if (out)
+            for (long i = 0; i < std::min<long>(cap, (long)L.size()); i++) out[i] = L[i];
+        return (long)L.size();
+    });
+}
+
 // ---- standalone linear algebra
 
 int cuibm_csr_create(int n, const int *rowptr, const int *col, const double *val, cuibm_csr **out) {
@@ -337,10 +347,10 @@ int cuibm_csr_spmv(cuibm_csr *A, const double *x, double *y) {
 }
 
 static AmgPrecond &amg_of(cuibm_csr *A, int precision) {
-    const int k = precision == 32 ? 1 : 0;
+    const int k = precision == 32 ? 1 : precision == 3264 ? 2 : 0;
     if (!A->amg[k]) {
         AmgParams p;
-        p.precision = precision == 32 ? 32 : 64;
+        p.precision = precision == 32 || precision == 3264 ? precision : 64;
         A->amg[k].reset(new AmgPrecond(A->h, p, A->stream));
     }
     return *A->amg[k];
@@ -367,6 +377,26 @@ int cuibm_bicgstab(cuibm_csr *A, double *x, const double *b, int precond, int am
         if (iters) *iters = st.iterations;
         if (resid) *resid = st.residual;
         return st.converged ? 0 : (int)ERR_NOT_CONVERGED;
+    });
+}
+
+int cuibm_set_body_velocity(cuibm_solver *s, const double *ub, const double *vb, int n, double omega) {
+    return guard([&] { need(s);
+        auto *l = dynamic_cast<luo_base *>(s->s.get());
+        if (!l) throw Error("per-point body velocities need an immersed-body solver", ERR_UNSUPPORTED);
+        if (!ub || !vb) throw Error("null body velocity arrays", ERR_ARGUMENT);
+        l->setBodyVelocity(ub, vb, n, omega);
+        return 0;
+    });
+}
+
+int cuibm_adjoint_run(cuibm_solver *s, int nsteps, int n0, int n1, int checkpoint_every, int backward, double *J,
+                      double *grad_ub, double *grad_vb, double *grad_omega, double *stats) {
+    return guard([&] { need(s);
+        auto *l = dynamic_cast<luoIBM *>(s->s.get());
+        if (!l) throw Error("the discrete adjoint is built for the LUO solver", ERR_UNSUPPORTED);
+        l->adjointRun(nsteps, n0, n1, checkpoint_every, backward != 0, J, grad_ub, grad_vb, grad_omega, stats);
+        return 0;
     });
 }
 

@@ -85,7 +85,9 @@ void parseSimulation(const YamlNode &node, ParameterDB &DB) {
         if (s.has("maxIterations")) maxIter = s["maxIterations"].as_int();
         if (s.has("amgPrecision")) {
             const std::string a = s["amgPrecision"].as_string();
-            amgPrec = (a == "FP32" || a == "32" || a == "float") ? 32 : 64;
+            if (a == "FP32" || a == "32" || a == "float") amgPrec = 32;
+            else if (a == "FP32VALUES" || a == "3264") amgPrec = 3264;
+            else amgPrec = 64;
         }
         p.preconditioner = preconditionerFromString(prec);
         p.tolerance = tol;
@@ -294,7 +296,11 @@ void readInputs(const std::string &caseFolder, int argc, const char *const *argv
         else if (a == "-dt") { DB.dt = std::atof(v); i++; }
         else if (a == "-velocityTol") { DB.velocitySolve.tolerance = std::atof(v); i++; }
         else if (a == "-poissonTol") { DB.PoissonSolve.tolerance = std::atof(v); i++; }
-        else if (a == "-amgPrecision") { DB.PoissonSolve.amgPrecision = std::atoi(v) == 32 ? 32 : 64; i++; }
+        else if (a == "-amgPrecision") {
+            const int prec = std::atoi(v);
+            DB.PoissonSolve.amgPrecision = (prec == 32 || prec == 3264) ? prec : 64;
+            i++;
+        }
         else if (a == "-poissonPreconditioner") { DB.PoissonSolve.preconditioner = preconditionerFromString(v); i++; }
         else if (a == "-operators") {
             const std::string o = v;

@@ -31,7 +31,9 @@ struct LinearSolverParams {
     PreconditionerType preconditioner = DIAGONAL;
     double tolerance = 1e-5;
     int maxIterations = 10000;
-    int amgPrecision = 64;      // 32: smoothed-aggregation hierarchy and V-cycle in FP32
+    // 64: FP64 smoothed-aggregation hierarchy; 32: V-cycle in FP32 (matrix values, level
+    // vectors, smoother arithmetic); 3264: FP32 matrix values with FP64 vectors
+    int amgPrecision = 64;
 };
 
 // typed restatement of the parameterDB filled by src/io/io.cu:145-197 and the

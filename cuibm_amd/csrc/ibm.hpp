@@ -64,6 +64,7 @@ struct Stencils {
     DArray<double> nrm;           // Neumann corner normals (nx, ny) per corner: 8n
     DArray<double> extra;         // per node: alpha (hybrid nodes) or dist/close-normal (pressure ghost: 3n)
     DArray<double> tmp;           // staging for the ghost-node velocity update
+    DArray<double> ub;            // per-node body velocity at the node's intercept (empty: uB[0] for all)
 };
 
 struct IbmGeom {
@@ -77,7 +78,9 @@ void launch_hn_p_coef(const GridDev &g, const IbmGeom &m, Stencils &st, hipStrea
 void launch_gn_p_coef(const GridDev &g, const IbmGeom &m, Stencils &st, hipStream_t s);
 
 void launch_gn_vel_apply(double *f, Stencils &st, int numU, double uB, double vB, hipStream_t s);
-void launch_set_inside(double *f, const int *solid, int nsolid, int numU, double uB, double vB, hipStream_t s);
+// solid nodes take uB / vB, or ubn[t] for the t-th solid node when ubn is given
+void launch_set_inside(double *f, const int *solid, int nsolid, int numU, double uB, double vB, hipStream_t s,
+                       const double *ubn = nullptr);
 void launch_hn_vel_apply(const double *f, double *ustar, const Stencils &st, int numU, double uB, double vB,
                          hipStream_t s);
 void launch_weight(double *f, const double *star, const Stencils &st, hipStream_t s);

@@ -684,10 +684,10 @@ __global__ void k_gn_p_coef(GridDev g, IbmGeom m, int n, const int *node, int *i
 
 __global__ void k_gn_vel_eval(const double *__restrict__ f, int n, const int *__restrict__ node,
                               const int *__restrict__ idx, const double *__restrict__ coef, double *__restrict__ tmp,
-                              int numU, double uB, double vB) {
+                              int numU, double uB, double vB, const double *__restrict__ ubn) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const double ub = node[t] < numU ? uB : vB;
+    const double ub = ubn ? ubn[t] : (node[t] < numU ? uB : vB);
     double s = 0;
     for (int l = 0; l < 4; l++) {
         const int j = idx[4 * t + l];
@@ -702,20 +702,21 @@ __global__ void k_scatter(double *__restrict__ f, int n, const int *__restrict__
 }
 
 __global__ void k_set_inside(double *__restrict__ f, const int *__restrict__ solid, int n, int numU, double uB,
-                             double vB) {
+                             double vB, const double *__restrict__ ubn) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) {
         const int s = solid[t];
-        f[s] = s < numU ? uB : vB;
+        f[s] = ubn ? ubn[t] : (s < numU ? uB : vB);
     }
 }
 
 __global__ void k_hn_vel_eval(const double *__restrict__ f, double *__restrict__ ustar, int n,
                               const int *__restrict__ node, const int *__restrict__ idx,
-                              const double *__restrict__ coef, int numU, double uB, double vB) {
+                              const double *__restrict__ coef, int numU, double uB, double vB,
+                              const double *__restrict__ ubn) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const double ub = node[t] < numU ? uB : vB;
+    const double ub = ubn ? ubn[t] : (node[t] < numU ? uB : vB);
     double s = 0;
     for (int l = 0; l < 4; l++) {
         const int j = idx[4 * t + l];
@@ -944,21 +945,22 @@ void launch_gn_p_coef(const GridDev &g, const IbmGeom &m, Stencils &st, hipStrea
 void launch_gn_vel_apply(double *f, Stencils &st, int numU, double uB, double vB, hipStream_t s) {
     if (!st.n) return;
     hipLaunchKernelGGL(k_gn_vel_eval, dim3(nblk(st.n)), dim3(BS), 0, s, f, st.n, st.node.p(), st.idx.p(), st.coef.p(),
-                       st.tmp.p(), numU, uB, vB);
+                       st.tmp.p(), numU, uB, vB, st.ub.empty() ? nullptr : (const double *)st.ub.p());
     CUIBM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scatter, dim3(nblk(st.n)), dim3(BS), 0, s, f, st.n, st.node.p(), st.tmp.p());
     CUIBM_LAUNCH_CHECK();
 }
-void launch_set_inside(double *f, const int *solid, int nsolid, int numU, double uB, double vB, hipStream_t s) {
+void launch_set_inside(double *f, const int *solid, int nsolid, int numU, double uB, double vB, hipStream_t s,
+                       const double *ubn) {
     if (!nsolid) return;
-    hipLaunchKernelGGL(k_set_inside, dim3(nblk(nsolid)), dim3(BS), 0, s, f, solid, nsolid, numU, uB, vB);
+    hipLaunchKernelGGL(k_set_inside, dim3(nblk(nsolid)), dim3(BS), 0, s, f, solid, nsolid, numU, uB, vB, ubn);
     CUIBM_LAUNCH_CHECK();
 }
 void launch_hn_vel_apply(const double *f, double *ustar, const Stencils &st, int numU, double uB, double vB,
                          hipStream_t s) {
     if (!st.n) return;
     hipLaunchKernelGGL(k_hn_vel_eval, dim3(nblk(st.n)), dim3(BS), 0, s, f, ustar, st.n, st.node.p(), st.idx.p(),
-                       st.coef.p(), numU, uB, vB);
+                       st.coef.p(), numU, uB, vB, st.ub.empty() ? nullptr : (const double *)st.ub.p());
     CUIBM_LAUNCH_CHECK();
 }
 void launch_weight(double *f, const double *star, const Stencils &st, hipStream_t s) {

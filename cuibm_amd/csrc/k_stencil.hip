@@ -141,28 +141,32 @@ __global__ void k_p5_dinv(P5Dev A, double *__restrict__ dinv) {
 }
 
 // the coupling magnitudes of the columns and rows one tile (with a halo of 2) touches,
-// staged in LDS; li, lj below are halo-2 tile coordinates (I = I0 - 2 + li)
+// staged in LDS in the compute type T; li, lj below are halo-2 tile coordinates
+// (I = I0 - 2 + li)
+template <class T>
 struct TileCoef {
-    double e[W2], w[W2], n[H2], s[H2];
+    T e[W2], w[W2], n[H2], s[H2];
 };
 
-__device__ __forceinline__ void load_coef(const P5Dev &A, int I0, int J0, TileCoef &c) {
+template <class T>
+__device__ __forceinline__ void load_coef(const P5Dev &A, int I0, int J0, TileCoef<T> &c) {
     for (int k = threadIdx.x; k < W2; k += blockDim.x) {
         const int I = I0 - 2 + k;
         const bool in = I >= 0 && I < A.nx;
-        c.e[k] = in ? A.cE[I] : 0.0;
-        c.w[k] = in ? A.cW[I] : 0.0;
+        c.e[k] = in ? (T)A.cE[I] : T(0);
+        c.w[k] = in ? (T)A.cW[I] : T(0);
     }
     for (int k = threadIdx.x; k < H2; k += blockDim.x) {
         const int J = J0 - 2 + k;
         const bool in = J >= 0 && J < A.ny;
-        c.n[k] = in ? A.cN[J] : 0.0;
-        c.s[k] = in ? A.cS[J] : 0.0;
+        c.n[k] = in ? (T)A.cN[J] : T(0);
+        c.s[k] = in ? (T)A.cS[J] : T(0);
     }
 }
 
-__device__ __forceinline__ double t_diag(const P5Dev &A, const TileCoef &c, int li, int lj, int I, int J, int ip) {
-    double t = 0;
+template <class T>
+__device__ __forceinline__ T t_diag(const P5Dev &A, const TileCoef<T> &c, int li, int lj, int I, int J, int ip) {
+    T t = 0;
     if (I != A.nx - 1) t += c.e[li];
     if (I != 0) t += c.w[li];
     if (J != A.ny - 1) t += c.n[lj];
@@ -172,10 +176,10 @@ __device__ __forceinline__ double t_diag(const P5Dev &A, const TileCoef &c, int 
     return t;
 }
 
-template <class F>
-__device__ __forceinline__ double t_row(const P5Dev &A, const TileCoef &c, int li, int lj, int I, int J, double d, F x,
-                                        int sy) {
-    double s = 0;
+template <class T, class F>
+__device__ __forceinline__ T t_row(const P5Dev &A, const TileCoef<T> &c, int li, int lj, int I, int J, T d, F x,
+                                   int sy) {
+    T s = 0;
     if (J != 0) s += (-c.s[lj]) * x(-sy);
     if (I != 0) s += (-c.w[li]) * x(-1);
     s += d * x(0);
@@ -184,29 +188,41 @@ __device__ __forceinline__ double t_row(const P5Dev &A, const TileCoef &c, int l
     return s;
 }
 
+// D^-1 in the compute type: the FP64 array, or its FP32 copy for the FP32 V-cycle
+template <class T>
+__device__ __forceinline__ const T *dinv_of(const P5Dev &A);
+template <>
+__device__ __forceinline__ const double *dinv_of<double>(const P5Dev &A) { return A.dinv; }
+template <>
+__device__ __forceinline__ const float *dinv_of<float>(const P5Dev &A) { return A.dinv32; }
+
 constexpr int N2 = H2 * W2, N1 = H1 * W1, R2 = (N2 + 255) / 256, R1 = (N1 + 255) / 256;
 
-// pre-smoothing (two sweeps from x = 0) and residual on one tile; all global loads
+// pre-smoothing (two sweeps from x = 0) and residual on one tile, in the compute type T
+// (b is the FP64 Krylov vector; x2 and r are level vectors of type T); all global loads
 // of a thread are issued before the first is used
-__global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double *__restrict__ b,
-                                                double *__restrict__ x2g, double *__restrict__ rg, int tiles_x,
-                                                int ntiles) {
+template <class T>
+__global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, T w, const double *__restrict__ b, T *__restrict__ x2g,
+                                                T *__restrict__ rg, int tiles_x, int ntiles) {
     const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
-    __shared__ TileCoef cf;
-    __shared__ double sb[N2], swd[N2], sx2[N1];
+    __shared__ TileCoef<T> cf;
+    __shared__ T sb[N2], swd[N2], sx2[N1];
     const int t = xcd_band(ntiles);
     if (t >= ntiles) return;  // uniform per block
     const int I0 = (t % tiles_x) * TX, J0 = (t / tiles_x) * TY;
     load_coef(A, I0, J0, cf);
-    double bv[R2], dv[R2];
+    const T *dinv = dinv_of<T>(A);
+    T bv[R2], dv[R2];
 #pragma unroll
     for (int m = 0; m < R2; m++) {
         const int k = threadIdx.x + 256 * m;
         const int I = I0 - 2 + k % W2, J = J0 - 2 + k / W2;
         const bool in = k < N2 && inside(A, I, J);
         const int ip = in ? J * A.nx + I : 0;
-        bv[m] = in ? b[ip] : 0.0;
-        dv[m] = in ? A.dinv[ip] : 0.0;
+        double bb = in ? b[ip] : 0.0;
+        if (A.defl && in && (ip == 0 || ip == A.centre)) bb -= A.defl[0] * (ip == 0 ? A.defl_a0 : A.defl_ac);
+        bv[m] = (T)bb;
+        dv[m] = in ? dinv[ip] : T(0);
     }
     if (stopv != 0.0) return;  // uniform across the block, before the first barrier
 #pragma unroll
@@ -221,11 +237,11 @@ __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double 
     for (int k = threadIdx.x; k < N1; k += blockDim.x) {
         const int li = k % W1 + 1, lj = k / W1 + 1;
         const int I = I0 - 2 + li, J = J0 - 2 + lj;
-        double v = 0;
+        T v = 0;
         if (inside(A, I, J)) {
             const int c = lj * W2 + li;
-            const double d = t_diag(A, cf, li, lj, I, J, J * A.nx + I);
-            const double s = t_row(A, cf, li, lj, I, J, d, [&](int o) { return swd[c + o] * sb[c + o]; }, W2);
+            const T d = t_diag(A, cf, li, lj, I, J, J * A.nx + I);
+            const T s = t_row(A, cf, li, lj, I, J, d, [&](int o) { return swd[c + o] * sb[c + o]; }, W2);
             v = swd[c] * sb[c] + swd[c] * (sb[c] - s);
         }
         sx2[k] = v;
@@ -237,31 +253,34 @@ __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, double w, const double 
         if (!inside(A, I, J)) continue;
         const int ip = J * A.nx + I;
         const int c1 = (lj + 1) * W1 + li + 1, c2 = (lj + 2) * W2 + li + 2;
-        const double d = t_diag(A, cf, li + 2, lj + 2, I, J, ip);
-        const double s = t_row(A, cf, li + 2, lj + 2, I, J, d, [&](int o) { return sx2[c1 + o]; }, W1);
+        const T d = t_diag(A, cf, li + 2, lj + 2, I, J, ip);
+        const T s = t_row(A, cf, li + 2, lj + 2, I, J, d, [&](int o) { return sx2[c1 + o]; }, W1);
         x2g[ip] = sx2[c1];
         rg[ip] = sb[c2] - s;
     }
 }
 
-// two weighted-Jacobi sweeps on one tile: x (halo 2) -> y (halo 1) -> xo (tile)
-__global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, double w, const double *__restrict__ x,
+// two weighted-Jacobi sweeps on one tile in the compute type T: x (halo 2, a level
+// vector) -> y (halo 1) -> xo (tile, the FP64 Krylov vector); b is the FP64 Krylov vector
+template <class T>
+__global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, T w, const T *__restrict__ x,
                                                    const double *__restrict__ b, double *__restrict__ xo, int tiles_x,
                                                    int ntiles) {
     const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
-    __shared__ TileCoef cf;
-    __shared__ double sx[N2], sb[N1], swd[N1];
+    __shared__ TileCoef<T> cf;
+    __shared__ T sx[N2], sb[N1], swd[N1];
     const int t = xcd_band(ntiles);
     if (t >= ntiles) return;
     const int I0 = (t % tiles_x) * TX, J0 = (t / tiles_x) * TY;
     load_coef(A, I0, J0, cf);
-    double xv[R2], bv[R1], dv[R1];
+    const T *dinv = dinv_of<T>(A);
+    T xv[R2], bv[R1], dv[R1];
 #pragma unroll
     for (int m = 0; m < R2; m++) {
         const int k = threadIdx.x + 256 * m;
         const int I = I0 - 2 + k % W2, J = J0 - 2 + k / W2;
         const bool in = k < N2 && inside(A, I, J);
-        xv[m] = in ? x[J * A.nx + I] : 0.0;
+        xv[m] = in ? x[J * A.nx + I] : T(0);
     }
 #pragma unroll
     for (int m = 0; m < R1; m++) {
@@ -269,9 +288,12 @@ __global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, double w, const doub
         const int I = I0 - 1 + k % W1, J = J0 - 1 + k / W1;
         const bool in = k < N1 && inside(A, I, J);
         const int ip = in ? J * A.nx + I : 0;
-        bv[m] = in ? b[ip] : 0.0;
-        dv[m] = in ? A.dinv[ip] : 0.0;
+        double bb = in ? b[ip] : 0.0;
+        if (A.defl && in && (ip == 0 || ip == A.centre)) bb -= A.defl[0] * (ip == 0 ? A.defl_a0 : A.defl_ac);
+        bv[m] = (T)bb;
+        dv[m] = in ? dinv[ip] : T(0);
     }
+    const double cdefl = A.defl ? A.defl[0] : 0.0;
     if (stopv != 0.0) return;  // uniform across the block, before the first barrier
 #pragma unroll
     for (int m = 0; m < R2; m++) {
@@ -288,25 +310,25 @@ __global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, double w, const doub
     }
     __syncthreads();
     // first sweep on the halo-1 region, kept in registers until every read of sx is done
-    double yv[R1];
+    T yv[R1];
 #pragma unroll
     for (int m = 0; m < R1; m++) {
         const int k = threadIdx.x + 256 * m;
-        double v = 0;
+        T v = 0;
         if (k < N1) {
             const int li = k % W1 + 1, lj = k / W1 + 1;
             const int I = I0 - 2 + li, J = J0 - 2 + lj;
             if (inside(A, I, J)) {
                 const int c = lj * W2 + li;
-                const double d = t_diag(A, cf, li, lj, I, J, J * A.nx + I);
-                const double s = t_row(A, cf, li, lj, I, J, d, [&](int o) { return sx[c + o]; }, W2);
+                const T d = t_diag(A, cf, li, lj, I, J, J * A.nx + I);
+                const T s = t_row(A, cf, li, lj, I, J, d, [&](int o) { return sx[c + o]; }, W2);
                 v = sx[c] + swd[k] * (sb[k] - s);
             }
         }
         yv[m] = v;
     }
     __syncthreads();
-    double *sy = sx;  // reuse: the halo-1 iterate in halo-1 layout
+    T *sy = sx;  // reuse: the halo-1 iterate in halo-1 layout
 #pragma unroll
     for (int m = 0; m < R1; m++) {
         const int k = threadIdx.x + 256 * m;
@@ -319,10 +341,32 @@ __global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, double w, const doub
         if (!inside(A, I, J)) continue;
         const int ip = J * A.nx + I;
         const int c1 = (lj + 1) * W1 + li + 1;
-        const double d = t_diag(A, cf, li + 2, lj + 2, I, J, ip);
-        const double s = t_row(A, cf, li + 2, lj + 2, I, J, d, [&](int o) { return sy[c1 + o]; }, W1);
-        xo[ip] = sy[c1] + swd[c1] * (sb[c1] - s);
+        const T d = t_diag(A, cf, li + 2, lj + 2, I, J, ip);
+        const T s = t_row(A, cf, li + 2, lj + 2, I, J, d, [&](int o) { return sy[c1 + o]; }, W1);
+        xo[ip] = (double)(sy[c1] + swd[c1] * (sb[c1] - s)) + cdefl;
     }
+}
+
+// partial sums of w_i r_i, w_i = dx[I] dy[J], over a fixed grid of RED_GRID blocks
+__global__ void __launch_bounds__(BS) k_p5_wdot(P5Dev A, const double *__restrict__ r, double *__restrict__ part) {
+    const double stopv = A.stop ? *A.stop : 0.0;
+    if (stopv != 0.0) return;
+    const int n = A.nx * A.ny, blk = xcd_band(RED_GRID);
+    double acc[1] = {0};
+    for (int ip = blk * BS + threadIdx.x; ip < n; ip += RED_GRID * BS)
+        acc[0] += (A.dx[ip % A.nx] * A.dy[ip / A.nx]) * r[ip];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blk] = acc[0];
+}
+
+// one block: c = sum of the partials (fixed order) * inv_wa
+__global__ void __launch_bounds__(BS) k_p5_wdot_finish(const double *__restrict__ part, double *__restrict__ defl,
+                                                       double inv_wa, const double *__restrict__ stop) {
+    if (stop && *stop != 0.0) return;
+    double acc[1] = {0};
+    for (int k = threadIdx.x; k < RED_GRID; k += BS) acc[0] += part[k];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) defl[0] = acc[0] * inv_wa;
 }
 
 // one sweep xn = x + omega D^-1 (b - A x), a row per thread
@@ -348,7 +392,8 @@ __global__ void __launch_bounds__(BS) k_p5_apply(P5Dev A, const double *__restri
 
 }  // namespace
 
-PoissonStencil::PoissonStencil(const GridDev &g, double dt, bool centre_pin, hipStream_t s) : nx_(g.nx), ny_(g.ny) {
+PoissonStencil::PoissonStencil(const GridDev &g, double dt, bool centre_pin, hipStream_t s)
+    : nx_(g.nx), ny_(g.ny), dx_(g.dx), dy_(g.dy) {
     // LHS2.cu:169-172: the centre pin applies to an interior cell only
     const int c = (ny_ / 2) * nx_ + nx_ / 2, cI = c % nx_, cJ = c / nx_;
     const bool interior = cI > 0 && cI < nx_ - 1 && cJ > 0 && cJ < ny_ - 1;
@@ -527,24 +572,54 @@ static void tiles(const P5Dev &A, int &tx, int &nt) {
     nt = tx * ((A.ny + TY - 1) / TY);
 }
 
-void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s,
-                   KernelProbe *probe) {
+template <class T>
+static void p5_pre(const P5Dev &A, T omega, const double *b, T *x2, T *r, hipStream_t s, KernelProbe *probe) {
     int tx, nt;
     tiles(A, tx, nt);
     if (probe && probe->enabled()) {
         hipEvent_t e0, e1;
         probe->pair(e0, e1);
-        hipExtLaunchKernelGGL(k_p5_pre, dim3(xcd_grid(nt)), dim3(256), 0, s, e0, e1, 0, A, omega, b, x2, r, tx, nt);
+        hipExtLaunchKernelGGL(k_p5_pre<T>, dim3(xcd_grid(nt)), dim3(256), 0, s, e0, e1, 0, A, omega, b, x2, r, tx,
+                              nt);
     } else {
-        hipLaunchKernelGGL(k_p5_pre, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, b, x2, r, tx, nt);
+        hipLaunchKernelGGL(k_p5_pre<T>, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, b, x2, r, tx, nt);
     }
     CUIBM_LAUNCH_CHECK();
 }
 
-void launch_p5_sweep2(const P5Dev &A, double omega, const double *x, const double *b, double *xo, hipStream_t s) {
+template <class T>
+static void p5_sweep2(const P5Dev &A, T omega, const T *x, const double *b, double *xo, hipStream_t s) {
     int tx, nt;
     tiles(A, tx, nt);
-    hipLaunchKernelGGL(k_p5_sweep2, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, x, b, xo, tx, nt);
+    hipLaunchKernelGGL(k_p5_sweep2<T>, dim3(xcd_grid(nt)), dim3(256), 0, s, A, omega, x, b, xo, tx, nt);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s,
+                   KernelProbe *probe) {
+    p5_pre<double>(A, omega, b, x2, r, s, probe);
+}
+
+void launch_p5_pre(const P5Dev &A, float omega, const double *b, float *x2, float *r, hipStream_t s,
+                   KernelProbe *probe) {
+    if (!A.dinv32) throw Error("FP32 finest-level smoother without its FP32 D^-1", ERR_ARGUMENT);
+    p5_pre<float>(A, omega, b, x2, r, s, probe);
+}
+
+void launch_p5_sweep2(const P5Dev &A, double omega, const double *x, const double *b, double *xo, hipStream_t s) {
+    p5_sweep2<double>(A, omega, x, b, xo, s);
+}
+
+void launch_p5_sweep2(const P5Dev &A, float omega, const float *x, const double *b, double *xo, hipStream_t s) {
+    if (!A.dinv32) throw Error("FP32 finest-level smoother without its FP32 D^-1", ERR_ARGUMENT);
+    p5_sweep2<float>(A, omega, x, b, xo, s);
+}
+
+void launch_p5_deflate(const P5Dev &A, const double *r, double *part, double *defl, double inv_wa, hipStream_t s) {
+    if (!A.dx || !A.dy) throw Error("deflation needs the cell widths of the stencil operator", ERR_ARGUMENT);
+    hipLaunchKernelGGL(k_p5_wdot, dim3(RED_GRID), dim3(BS), 0, s, A, r, part);
+    CUIBM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_p5_wdot_finish, dim3(1), dim3(BS), 0, s, part, defl, inv_wa, A.stop);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -128,7 +128,10 @@ class JacobiPrecond : public Preconditioner {
 
 // smoothed-aggregation AMG (setup on the host, V-cycle on the device)
 struct AmgParams {
-    int precision = 64;   // 64 or 32
+    // 64: FP64 hierarchy; 32: matrix values, level vectors and smoother arithmetic in FP32
+    // (the V-cycle reads the FP64 Krylov vector and writes its result in FP64); 3264:
+    // FP32 matrix values with FP64 level vectors and arithmetic
+    int precision = 64;
     int coarse_max = 2500;  // dense coarsest solve (one 25 MB FP32 / 50 MB FP64 matrix at most)
     int max_levels = 25;
     double omega = 0.8;
@@ -203,6 +206,13 @@ struct P5Dev {
     const double *cE, *cW, *cN, *cS;   // coupling magnitudes; 0 where the neighbour is missing
     const double *dinv;                // 1 / diagonal, per cell
     const double *stop = nullptr;      // the kernels return at once while *stop != 0
+    const float *dinv32 = nullptr;     // the same in FP32 (V-cycle with FP32 level vectors)
+    const double *dx = nullptr, *dy = nullptr;  // cell widths (the cell areas weight the deflation)
+    // FP64 deflation of the pinned constant mode for the FP32 V-cycle: defl[0] = c, the
+    // input is r - c A 1 (A 1 is a0 at cell 0 and ac at the centre pin) and c is added to
+    // the output
+    const double *defl = nullptr;
+    double defl_a0 = 0, defl_ac = 0;
 };
 
 class AmgPrecond : public Preconditioner {
@@ -214,8 +224,14 @@ class AmgPrecond : public Preconditioner {
     ~AmgPrecond() override;
     void apply(const double *r, double *z, hipStream_t s) override;
     void set_stop(const double *stop) override;
-    std::string name() const override { return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)" : "SMOOTHED_AGGREGATION"; }
+    std::string name() const override {
+        return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)"
+                                    : prm_.precision == 3264 ? "SMOOTHED_AGGREGATION(FP32 values)" : "SMOOTHED_AGGREGATION";
+    }
     double bytes_per_apply() const override;
+    // algorithmic bytes of each launch of one V-cycle, in launch order (their sum is
+    // bytes_per_apply())
+    std::vector<double> launch_bytes() const;
     AmgHierarchyInfo info() const;
     // one weighted-Jacobi sweep xn = x + omega D^-1 (b - A x) on the finest level, as
     // run inside the V-cycle (for timing), and its algorithmic bytes

@@ -8,6 +8,7 @@
 // is reported by throwing cuibm::Error(ERR_NOT_CONVERGED) instead of exit(-1).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <iomanip>
 #include <sstream>
@@ -260,8 +261,10 @@ void NavierStokesSolver::solveIntermediateVelocity() {
 
 void NavierStokesSolver::solvePoisson() {
     ScopedTimer t(logger, "Poisson Solve");
+    poissonProbe.mark(stream_);
     stat2 = ks2->solve(*op2, pressure.p(), rhs2.p(), *PC2, db->PoissonSolve.tolerance, db->PoissonSolve.maxIterations,
                        stream_);
+    poissonProbe.mark(stream_);
     iterationCount2 = stat2.iterations;
     itTotal2 += stat2.iterations;
     restartsTotal += stat2.restarts;
@@ -455,6 +458,28 @@ static double krylov_bytes(const LinOp &A, const Preconditioner &M, const SolveS
 }
 
 double NavierStokesSolver::poissonBytes() const { return PC2 ? krylov_bytes(*op2, *PC2, stat2) : 0.0; }
+
+std::vector<double> NavierStokesSolver::poissonIterationLaunches() const {
+    std::vector<double> L;
+    const AmgPrecond *a = dynamic_cast<const AmgPrecond *>(PC2.get());
+    if (!a) return L;
+    const std::vector<double> vc = a->launch_bytes();
+    const double n = op2->n(), spmv = op2->spmv_bytes(), fin = 16.0 * RED_GRID;
+    // half 1: z = M p, y = A z with <rs, y>, finish, s = r - alpha y with <s, s>, finish
+    L.insert(L.end(), vc.begin(), vc.end());
+    L.push_back(spmv + 8 * n);
+    L.push_back(fin);
+    L.push_back(3 * 8 * n);
+    L.push_back(fin);
+    // half 2: z = M s, t = A z with <t, s>, <t, t>, finish, x and r with their dots, finish, p
+    L.insert(L.end(), vc.begin(), vc.end());
+    L.push_back(spmv + 8 * n);
+    L.push_back(fin);
+    L.push_back(8 * 8 * n);
+    L.push_back(fin);
+    L.push_back(4 * 8 * n);
+    return L;
+}
 double NavierStokesSolver::velocityBytes() const { return PC1 ? krylov_bytes(*op1, *PC1, stat1) : 0.0; }
 double NavierStokesSolver::stepBytes() const {
     // rhs1 sweep (u, Nold in; N, rhs1 out), rhs2 (uhat in, rhs2 out), projection (uhat, p, u in; u, uold out)
@@ -479,12 +504,32 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
         if (bytes) *bytes = a->level0_pre_bytes();
         return nl;
     }
+    if (k == "live:poisson") {  // every Poisson solve of `reps` live time steps, as run
+        const double b0 = bytesTotal2, n0 = solvesTotal2;
+        poissonProbe.enable(true);
+        for (int r = 0; r < reps; r++) stepTime();
+        const double t = poissonProbe.total_ms();
+        const int nl = poissonProbe.launches();
+        poissonProbe.enable(false);
+        const double solves = solvesTotal2 - n0;
+        if (ms) *ms = nl > 0 ? t / nl : 0.0;
+        if (bytes) *bytes = solves > 0 ? (bytesTotal2 - b0) / solves : 0.0;
+        return nl;
+    }
     DArray<double> t1(std::max(numUV, numP)), t2(std::max(numUV, numP)), part(2 * RED_GRID);
     // calibration copy: 64 MiB per array, larger than the L2 of every XCD together, so
     // that back-to-back launches cannot serve reads from L2
-    const long kCopyN = 8l << 20;
+    // "copy:N" copies N doubles instead (the size curve of a plain copy)
+    long kCopyN = 8l << 20;
+    const bool isCopy = k == "copy" || k.rfind("copy:", 0) == 0;
+    if (k.rfind("copy:", 0) == 0) {
+        char *end = nullptr;
+        kCopyN = std::strtol(k.c_str() + 5, &end, 10);
+        if (!end || *end != 0 || kCopyN < 1 || kCopyN > (1l << 28))
+            throw Error("copy size must be 1 .. 2^28 doubles: '" + k + "'", ERR_ARGUMENT);
+    }
     DArray<double> c1, c2;
-    if (k == "copy") {
+    if (isCopy) {
         c1.alloc(kCopyN);
         c2.alloc(kCopyN);
         launch_fill(c1.p(), 1.0, kCopyN, s);
@@ -493,7 +538,7 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
     auto once = [&]() {
         if (k == "spmv_lhs2") launch_spmv<double, double>(LHS2, pressure.p(), t1.p(), s);
         else if (k == "spmv_dot_lhs2") op2->spmv_dot(pressure.p(), t1.p(), rhs2.p(), true, part.p(), s);
-        else if (k == "copy") launch_copy(c2.p(), c1.p(), kCopyN, s);
+        else if (isCopy) launch_copy(c2.p(), c1.p(), kCopyN, s);
         else if (k == "spmv_lhs1") launch_spmv<double, double>(LHS1, uhat.p(), t1.p(), s);
         else if (k == "rhs1")
             launch_rhs1(u.p(), Nold.p(), t1.p(), t2.p(), nullptr, nullptr, grid(), bcs(), nu, dt, fadlunRhs(), s);
@@ -505,7 +550,7 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
     };
     if (k == "spmv_lhs2") b = csr_bytes(LHS2);
     else if (k == "spmv_dot_lhs2") b = op2->spmv_bytes() + 8.0 * numP + 16.0 * RED_GRID;  // + dot operand, partials
-    else if (k == "copy") b = 16.0 * kCopyN;
+    else if (isCopy) b = 16.0 * kCopyN;
     else if (k == "spmv_lhs1") b = csr_bytes(LHS1);
     else if (k == "rhs1") b = 32.0 * numUV + 8.0 * (2 * nx + 2 * ny) + 4 * 8.0 * (2 * ny + 2 * nx);
     else if (k == "vcycle") b = PC2 ? PC2->bytes_per_apply() : 0;
@@ -705,6 +750,10 @@ void luo_base::buildStencils() {
     setup(hnV, lh, 1);
     setup(gnP, lgp, 3);
     setup(hnP, lhp, 1);
+    // per-point body velocities refer to the node lists of one tagging
+    gnV.ub.release();
+    hnV.ub.release();
+    solidUB.release();
     solid.upload(ls, s);
     if (ls.empty()) solid.alloc(1);
     nsolid = (int)ls.size();
@@ -763,7 +812,7 @@ void luo_base::_project_velocity() {
     ScopedTimer t(logger, "Velocity Projection");
     projectPlain();
     launch_gn_vel_apply(u.p(), gnV, numU, bUB[0], bVB[0], stream_);
-    launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_);
+    launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_, solidUB.empty() ? nullptr : solidUB.p());
 }
 
 // luo_base.cu:255-264
@@ -855,6 +904,91 @@ void luo_base::readForces() {
 }
 
 void luo_base::updateRobinBoundary() { launch_robin(u.p(), bc[XPLUS].p(), grid(), dt, stream_); }
+
+void luo_base::setBodyVelocity(const double *ub, const double *vb, int n, double omega) {
+    if (!tagged) throw Error("setBodyVelocity needs an initialised solver", ERR_ARGUMENT);
+    if (moving) throw Error("per-point body velocities are for static bodies", ERR_UNSUPPORTED);
+    if (n != (int)bX.size()) throw Error("setBodyVelocity: " + std::to_string(n) + " values for " +
+                                             std::to_string(bX.size()) + " body points", ERR_ARGUMENT);
+    hipStream_t s = stream_;
+    const std::vector<double> bx = bi_x.to_host(s), by = bi_y.to_host(s);
+    const std::vector<double> hx = gx.to_host(s), hy = gy.to_host(s), hxu = gxu.to_host(s), hyv = gyv.to_host(s);
+    const int np = n;
+    // the body segment (k, k+1) whose line holds the intercept, and the intercept's
+    // position along it (the foot of a perpendicular onto that segment's line)
+    auto segment = [&](double px, double py, int &seg, double &w) {
+        double best = 1e300;
+        for (int k = 0; k < np; k++) {
+            const int l = (k + 1) % np;
+            const double ex = bX[l] - bX[k], ey = bY[l] - bY[k];
+            const double len2 = ex * ex + ey * ey;
+            double t = len2 > 0 ? ((px - bX[k]) * ex + (py - bY[k]) * ey) / len2 : 0.0;
+            t = std::min(1.0, std::max(0.0, t));
+            const double qx = bX[k] + t * ex - px, qy = bY[k] + t * ey - py;
+            const double d = qx * qx + qy * qy;
+            if (d < best) {
+                best = d;
+                seg = k;
+                w = t;
+            }
+        }
+    };
+    auto nodeValues = [&](const Stencils &st, std::vector<int> &nodes, std::vector<int> &seg, std::vector<double> &wt,
+                          std::vector<double> &val) {
+        nodes = st.n ? st.node.to_host(s) : std::vector<int>();
+        nodes.resize(st.n);
+        seg.assign(st.n, 0);
+        wt.assign(st.n, 0.0);
+        val.assign(std::max(st.n, 1), 0.0);
+        for (int t = 0; t < st.n; t++) {
+            const int i = nodes[t];
+            segment(bx[i], by[i], seg[t], wt[t]);
+            const int k = seg[t], l = (k + 1) % np;
+            const double *c = i < numU ? ub : vb;
+            val[t] = (1 - wt[t]) * c[k] + wt[t] * c[l];
+        }
+    };
+    std::vector<double> vg, vh;
+    nodeValues(gnV, gnNode, gnSeg, gnW, vg);
+    nodeValues(hnV, hnNode, hnSeg, hnW, vh);
+    gnV.ub.upload(vg, s);
+    hnV.ub.upload(vh, s);
+    // solid nodes: rigid rotation about the body centre
+    double cx = 0, cy = 0;
+    for (int k = 0; k < np; k++) {
+        cx += bX[k];
+        cy += bY[k];
+    }
+    cx /= np;
+    cy /= np;
+    solidNode = nsolid ? solid.to_host(s) : std::vector<int>();
+    solidNode.resize(nsolid);
+    std::vector<double> vs(std::max(nsolid, 1), 0.0);
+    for (int t = 0; t < nsolid; t++) {
+        const int i = solidNode[t];
+        if (i < numU) {
+            const int J = i / (nx - 1);
+            vs[t] = -omega * (hy[J] - cy);
+        } else {
+            const int I = (i - numU) % nx;
+            vs[t] = omega * (hx[I] - cx);
+        }
+    }
+    (void)hxu;
+    (void)hyv;
+    solidUB.upload(vs, s);
+    for (int k = 0; k < np; k++) {
+        bUB[k] = ub[k];
+        bVB[k] = vb[k];
+    }
+    dbu.upload(bUB, s);
+    dbv.upload(bVB, s);
+    perPointUB = true;
+    bodyOmega = omega;
+    // the initial field's solid nodes take the new velocities (luoIBM.cu:27-52)
+    launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], s, solidUB.p());
+    CUIBM_HIP(hipStreamSynchronize(s));
+}
 
 // luo_base.cu:119-128
 void luo_base::writeData() {
@@ -1015,7 +1149,9 @@ void luoIBM::rhs1GNInterpolation() {
 
 void luoIBM::rhs1HNInterpolation() { launch_hn_vel_apply(u.p(), ustar.p(), hnV, numU, bUB[0], bVB[0], stream_); }
 
-void luoIBM::zeroVelocity() { launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_); }
+void luoIBM::zeroVelocity() {
+    launch_set_inside(u.p(), solid.p(), nsolid, numU, bUB[0], bVB[0], stream_, solidUB.empty() ? nullptr : solidUB.p());
+}
 
 void luoIBM::weightUhat() {
     ScopedTimer t(logger, "Intermediate Velocity Weight");

@@ -64,6 +64,9 @@ class NavierStokesSolver {
     std::string preconditionerName(int which) const;
     // mean time of `reps` launches of one kernel on the live state (HIP events)
     int timeKernel(const std::string &kernel, int reps, double *ms, double *bytes);
+    // algorithmic bytes of each launch of one Poisson BiCGSTAB iteration, in launch order
+    // (empty unless the Poisson preconditioner is the AMG V-cycle)
+    std::vector<double> poissonIterationLaunches() const;
     // LHS2 rows follow the plain formula (no IBM row overrides): the Poisson solve
     // can use the stencil form of the operator
     virtual bool stencilLHS2() const { return !db->csrOperators; }
@@ -85,6 +88,7 @@ class NavierStokesSolver {
     double itTotal1 = 0, itTotal2 = 0, bytesTotal1 = 0, bytesTotal2 = 0, solvesTotal2 = 0, restartsTotal = 0;
     SolveStat stat1, stat2;
     Logger logger;  // the reference's named phase timers
+    KernelProbe poissonProbe;  // events around each Poisson solve while enabled (live:poisson)
     double cfl_max = 0;
     int cfl_I = 0, cfl_J = 0, cfl_ts = 0;
     DArray<double> cflPart, cflOut;
@@ -145,6 +149,11 @@ class luo_base : public NavierStokesSolver {
     void tagPoints();
     void calculateForce();
     void updateRobinBoundary();
+    // per-point body velocities ub[k], vb[k] of the body points and the angular velocity
+    // omega of the rigid rotation given to the solid nodes (a build extension: the
+    // reference gives every point the velocity uB[0]); static bodies only, after
+    // initialise()
+    void setBodyVelocity(const double *ub, const double *vb, int n, double omega);
 
     int phase(const std::string &name) override;
     long fieldCount(const std::string &name) const override;
@@ -173,6 +182,16 @@ class luo_base : public NavierStokesSolver {
     Stencils gnV, hnV, hnP, gnP;
     DArray<int> solid;
     int nsolid = 0;
+    // per-point body velocities (setBodyVelocity): the velocity at each ghost / hybrid
+    // node's body intercept, linear along the body segment that holds it, and rigid
+    // rotation at the solid nodes; the segment and its weight per node are kept for the
+    // adjoint's parameter gradient
+    DArray<double> solidUB;
+    std::vector<int> gnSeg, hnSeg;      // first point of the segment holding each node's intercept
+    std::vector<double> gnW, hnW;       // the intercept's weight on the segment's second point
+    std::vector<int> gnNode, hnNode, solidNode;
+    bool perPointUB = false;
+    double bodyOmega = 0;
     std::ofstream forceFile, midPositionFile;
     std::ofstream forceFullFile;  // the forces at 17 significant digits (no reference counterpart)
     bool tagged = false;
@@ -204,8 +223,22 @@ class luoIBM : public luo_base {
     std::string name() const override { return "LUO"; }
     int phase(const std::string &name) override;
 
+    // Discrete adjoint of J = mean over steps n0..n1 (1-based) of C_D = 2 Fx, for a run of
+    // `nsteps` steps from the current state (time step 0, after initialise and
+    // setBodyVelocity): the forward run stores checkpoints every `every` steps, then
+    // segments are recomputed from their checkpoint and stepped backwards (adjoint.hip).
+    // Writes J and, when backward, dJ/dub[k], dJ/dvb[k] of the per-point body velocities
+    // and dJ/domega of the solid nodes' rotation; stats[0..5] = forward and backward
+    // seconds, forward Poisson and velocity iterations, adjoint Poisson and velocity
+    // iterations.
+    void adjointRun(int nsteps, int n0, int n1, int every, bool backward, double *J, double *gub, double *gvb,
+                    double *gomega, double *stats);
+    ~luoIBM() override;
+
   protected:
     bool centrePin() const override { return false; }
+    struct Adjoint;
+    std::shared_ptr<Adjoint> adj_;  // built on the first backward run (adjoint.hip)
 };
 
 // Luo et al. embedded method (luo_iter.h): the interpolation lives in the matrices

@@ -32,12 +32,18 @@ class PoissonStencil : public LinOp {
     void init(const double *x, const double *b, double *r, double *p, double *rs, double *part,
               hipStream_t s) const override;
     double spmv_bytes() const override { return 16.0 * n() + 32.0 * (nx_ + ny_); }
-    P5Dev dev() const { return P5Dev{nx_, ny_, centre_, cE_.p(), cW_.p(), cN_.p(), cS_.p(), dinv_.p(), stop_}; }
+    P5Dev dev() const {
+        P5Dev d{nx_, ny_, centre_, cE_.p(), cW_.p(), cN_.p(), cS_.p(), dinv_.p(), stop_};
+        d.dx = dx_;
+        d.dy = dy_;
+        return d;
+    }
     // y = A x without dot partials (tests)
     void apply(const double *x, double *y, hipStream_t s) const;
 
   private:
     int nx_, ny_, centre_;
+    const double *dx_, *dy_;
     DArray<double> cE_, cW_, cN_, cS_, dinv_;
 };
 
@@ -82,6 +88,15 @@ class VelocityStencil : public LinOp {
 void launch_p5_pre(const P5Dev &A, double omega, const double *b, double *x2, double *r, hipStream_t s,
                    KernelProbe *probe = nullptr);
 void launch_p5_sweep2(const P5Dev &A, double omega, const double *x, const double *b, double *xo, hipStream_t s);
+// the same in FP32 (the V-cycle with FP32 level vectors): b is the FP64 Krylov vector,
+// x2 and r are FP32 level vectors, the result xo is written as FP64; D^-1 is A.dinv32
+void launch_p5_pre(const P5Dev &A, float omega, const double *b, float *x2, float *r, hipStream_t s,
+                   KernelProbe *probe = nullptr);
+void launch_p5_sweep2(const P5Dev &A, float omega, const float *x, const double *b, double *xo, hipStream_t s);
+// c = (sum_i w_i r_i) * inv_wa with w_i the area of cell i, into defl[0] (fixed-order
+// partial sums over RED_GRID blocks in part, then one block); returns at once while the
+// Krylov solve's stop code is set
+void launch_p5_deflate(const P5Dev &A, const double *r, double *part, double *defl, double inv_wa, hipStream_t s);
 // one sweep xn = x + omega D^-1 (b - A x) (timing of the finest-level smoother)
 void launch_p5_sweep(const P5Dev &A, double omega, const double *x, const double *b, double *xn, hipStream_t s);
 

@@ -46,8 +46,11 @@ const char *cuibm_last_error(void);
 /* io::readInputs (src/io/io.cu:108-138) + the SolverType factory (src/cuIBM.cu:42-60):
  * read <case_folder>/{simParams,flow,domain,bodies}.yaml, apply the command-line
  * overrides in argv (-nu -uPerturb -vPerturb -scaleCV -nsave -nt -dt -velocityTol
- * -poissonTol, plus -amgPrecision 32|64, -poissonPreconditioner NAME, -operators csr|stencil,
- * -outputFolder DIR and -output 0|1) and construct the solver (not yet initialised). */
+ * -poissonTol, plus -amgPrecision 64|32|3264 (as cuibm_bicgstab's amg_precision; in a
+ * solver whose Poisson operator has its stencil form, 32 also removes the pinned constant
+ * mode in FP64 around the FP32 V-cycle), -poissonPreconditioner NAME,
+ * -operators csr|stencil, -outputFolder DIR and -output 0|1) and construct the solver
+ * (not yet initialised). */
 int cuibm_create(const char *case_folder, int argc, const char *const *argv, cuibm_solver **out);
 
 /* NavierStokesSolver::initialise() (NavierStokesSolver.cu:34-40, luoIBM.cu:27-52,
@@ -157,15 +160,51 @@ int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms);
  * sweep on the finest AMG level), "vc_pre" / "vc_post" (the fused finest-level
  * pre-smoothing + residual and prolongation + post-smoothing kernels of the stencil
  * form, as launched inside the V-cycle), "copy" (a copy of 8 Mi doubles,
- * the calibration kernel for hardware byte counters). "live:vc_pre" instead runs
+ * the calibration kernel for hardware byte counters), "copy:N" (a copy of N doubles,
+ * 1 <= N <= 2^28: the rate a plain copy reaches at a given launch size). "live:vc_pre" instead runs
  * `reps` time steps and returns the mean duration of the finest-level pre-smoothing
  * launches inside them (HIP events around each launch); the return value is then the
- * number of launches timed. */
+ * number of launches timed. "live:poisson" likewise returns the mean device time of one
+ * whole Poisson solve (events recorded before and after each solve on the solver's
+ * stream, host waits inside the solve included) over `reps` time steps, with *bytes the
+ * mean algorithmic bytes of one solve as run; the return value is the number of solves. */
 int cuibm_time_kernel(cuibm_solver *s, const char *kernel, int reps, double *ms, double *bytes);
+
+/* The algorithmic bytes of each kernel launch of one Poisson BiCGSTAB iteration with the
+ * AMG V-cycle (what = "poisson_iteration"), in launch order, into out (up to cap values);
+ * returns their number (0 without the AMG preconditioner). With the rate a plain copy
+ * reaches at each launch size ("copy:N" above) this gives the time the iteration would
+ * take if every launch ran at that rate. No reference counterpart. */
+long cuibm_launch_bytes(cuibm_solver *s, const char *what, double *out, long cap);
 
 /* block until all work on the current device has finished (hipDeviceSynchronize);
  * the timing harness brackets its timed region with it. No reference counterpart. */
 int cuibm_device_synchronize(void);
+
+/* ---- discrete adjoint (no reference counterpart) ----------------------------------- */
+
+/* Per-point body velocities, a build extension: the reference gives every body point the
+ * velocity uB[0] (luo_base/kernels/biLinearInterpolation.cu:87,181). Here ub[k], vb[k]
+ * (n = number of body points) are interpolated linearly along the body segment that holds
+ * each ghost or hybrid node's body intercept, and the solid nodes take the rigid rotation
+ * u = -omega (y - yc), v = omega (x - xc) about the body centre. Static immersed bodies
+ * (LUO), after cuibm_initialise; the initial field's solid nodes are updated. */
+int cuibm_set_body_velocity(cuibm_solver *s, const double *ub, const double *vb, int n, double omega);
+
+/* Discrete adjoint of J = (1 / (n1 - n0 + 1)) sum_{n = n0..n1} 2 Fx^n (mean C_D over
+ * steps n0..n1, 1-based) for a run of nsteps steps from a freshly initialised LUO solver
+ * with per-point body velocities: the forward run keeps checkpoints every
+ * checkpoint_every steps; with backward != 0 each segment is recomputed from its
+ * checkpoint and stepped backwards (transposed LHS1 / LHS2 solves, transposed
+ * interpolation, transposed stencils). Writes *J; with backward, grad_ub[k] = dJ/dub[k],
+ * grad_vb[k] = dJ/dvb[k] (arrays of n) and *grad_omega = dJ/domega; stats (may be null,
+ * 8 values): forward seconds, backward seconds, forward Poisson and velocity iterations,
+ * adjoint Poisson and velocity iterations, the backward pass's set-up seconds (transposed
+ * operators, their hierarchy, the inverse stencil maps) and its seconds recomputing the
+ * segments. The solver's state afterwards is not the
+ * state after nsteps steps (the backward pass recomputes earlier segments). */
+int cuibm_adjoint_run(cuibm_solver *s, int nsteps, int n0, int n1, int checkpoint_every, int backward, double *J,
+                      double *grad_ub, double *grad_vb, double *grad_omega, double *stats);
 
 /* ---- ensembles of independent cases ------------------------------------------------ */
 
@@ -192,8 +231,10 @@ void cuibm_csr_destroy(cuibm_csr *A);
 int cuibm_csr_spmv(cuibm_csr *A, const double *x, double *y);
 /* cusp::krylov::bicgstab(A, x, b, cusp::monitor(b, maxit, rtol), M): x (host) is the
  * warm start and the result; precond = 0 NONE, 1 DIAGONAL, 2 SMOOTHED_AGGREGATION
- * (preconditioner.h:23-29); amg_precision = 64, or 32 for a hierarchy whose matrix
- * values are stored in FP32 (vectors and arithmetic stay FP64). Returns 0 when converged,
+ * (preconditioner.h:23-29); amg_precision = 64 (FP64 hierarchy), 32 (the V-cycle in FP32:
+ * matrix values, level vectors and smoother arithmetic; it reads the FP64 residual and
+ * returns an FP64 correction) or 3264 (FP32 matrix values with FP64 vectors and
+ * arithmetic). Returns 0 when converged,
  * CUIBM_ERR_NOT_CONVERGED otherwise, with *iters and *resid set either way. */
 int cuibm_bicgstab(cuibm_csr *A, double *x, const double *b, int precond, int amg_precision, double rtol, int maxit,
                    int *iters, double *resid);

@@ -169,6 +169,10 @@ struct Solver {
     void robin();
     void gn_velocity(std::vector<double> &f, bool set);
     void set_inside(std::vector<double> &f);
+    // per-point body velocities (no reference counterpart): per velocity node, the body
+    // value its ghost / hybrid interpolation and the solid-node update use; empty = uB[0]
+    std::vector<double> node_ub;
+    void set_body_velocity(const double *ub, const double *vb, double omega);
     void hn_velocity(const std::vector<double> &f);
     void weight_uhat();
     void hn_pressure();

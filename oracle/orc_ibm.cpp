@@ -471,6 +471,7 @@ void Solver::gn_velocity(std::vector<double> &f, bool set) {
                 int index[4] = {(jj - 1) * C.stride + ii - 1 + C.off, (jj - 1) * C.stride + ii + C.off,
                                 jj * C.stride + ii - 1 + C.off, jj * C.stride + ii + C.off};
                 double q[4] = {f[index[0]], f[index[1]], f[index[2]], f[index[3]]};
+                const double ubg = node_ub.empty() ? ubody[c] : node_ub[g];
                 double mn = 1.0;
                 int close_index = -1;
                 bool inflag = false;
@@ -483,11 +484,11 @@ void Solver::gn_velocity(std::vector<double> &f, bool set) {
                     if (ghostTagsUV[index[l]] == g) {
                         x[l] = bi_x[index[l]];
                         y[l] = bi_y[index[l]];
-                        q[l] = ubody[c];
+                        q[l] = ubg;
                     } else if (index[l] == close_index && !inflag) {
                         x[l] = bi_x[g];
                         y[l] = bi_y[g];
-                        q[l] = ubody[c];
+                        q[l] = ubg;
                     }
                 }
                 double a[16];
@@ -504,7 +505,7 @@ void Solver::gn_velocity(std::vector<double> &f, bool set) {
                     qval[l][g] = q[l];
                 }
                 image_point_u[g] = av[0] + av[1] * X + av[2] * Y + av[3] * X * Y;
-                if (set) out.push_back({g, 2 * ubody[c] - image_point_u[g]});
+                if (set) out.push_back({g, 2 * ubg - image_point_u[g]});
             }
         for (auto &e : out) f[e.first] = e.second;
     }
@@ -516,9 +517,59 @@ void Solver::set_inside(std::vector<double> &f) {
     for (int i = 0; i < numU; i++) {
         const int I = i % (nx - 1), J = i / (nx - 1);
         const int iu = J * (nx - 1) + I, iv = J * nx + I + (nx - 1) * ny;
-        f[iu] = (ghostTagsUV[iu] != 0) * f[iu] + (ghostTagsUV[iu] == 0) * B.uB[0];
-        if (iv < numUV) f[iv] = (ghostTagsUV[iv] != 0) * f[iv] + (ghostTagsUV[iv] == 0) * B.vB[0];
+        const double bu = node_ub.empty() ? B.uB[0] : node_ub[iu];
+        f[iu] = (ghostTagsUV[iu] != 0) * f[iu] + (ghostTagsUV[iu] == 0) * bu;
+        if (iv < numUV) {
+            const double bv = node_ub.empty() ? B.vB[0] : node_ub[iv];
+            f[iv] = (ghostTagsUV[iv] != 0) * f[iv] + (ghostTagsUV[iv] == 0) * bv;
+        }
     }
+}
+
+// Per-point body velocities, a build extension with no reference counterpart (the
+// reference gives every point uB[0], biLinearInterpolation.cu:87,181): a ghost or hybrid
+// node takes the value interpolated linearly along the body segment (k, k+1) whose
+// nearest point to the node's body intercept is closest (the first such segment on a
+// tie), and a node inside the body (tag 0) the rigid rotation omega about the mean of the
+// body points, u = -omega (y - yc), v = omega (x - xc), at the node's position. The
+// solid nodes of u are updated at once.
+void Solver::set_body_velocity(const double *ub, const double *vb, double omega) {
+    const int np = B.n;
+    node_ub.assign(numUV, 0.0);
+    double cx = 0, cy = 0;
+    for (int k = 0; k < np; k++) {
+        cx += B.x[k];
+        cy += B.y[k];
+    }
+    cx /= np;
+    cy /= np;
+    for (int i = 0; i < numUV; i++) {
+        const bool isU = i < numU;
+        if (ghostTagsUV[i] > 0 || hybridTagsUV[i] > 0) {
+            const double px = bi_x[i], py = bi_y[i];
+            double best = 1e300, wbest = 0;
+            int kbest = 0;
+            for (int k = 0; k < np; k++) {
+                const int l = (k + 1) % np;
+                const double ex = B.x[l] - B.x[k], ey = B.y[l] - B.y[k];
+                const double len2 = ex * ex + ey * ey;
+                double t = len2 > 0 ? ((px - B.x[k]) * ex + (py - B.y[k]) * ey) / len2 : 0.0;
+                t = std::min(1.0, std::max(0.0, t));
+                const double d = sq(B.x[k] + t * ex - px) + sq(B.y[k] + t * ey - py);
+                if (d < best) {
+                    best = d;
+                    kbest = k;
+                    wbest = t;
+                }
+            }
+            const double *c = isU ? ub : vb;
+            node_ub[i] = (1 - wbest) * c[kbest] + wbest * c[(kbest + 1) % np];
+        } else if (ghostTagsUV[i] == 0) {
+            if (isU) node_ub[i] = -omega * (G.y[i / (nx - 1)] - cy);
+            else node_ub[i] = omega * (G.x[(i - numU) % nx] - cx);
+        }
+    }
+    set_inside(u);
 }
 
 // interpolateVelocityToHybridNodeX/Y (biLinearInterpolation.cu:356-668)
@@ -545,7 +596,7 @@ void Solver::hn_velocity(const std::vector<double> &f) {
                     if (hybridTagsUV[index[l]] == h) {
                         x[l] = bi_x[h];
                         y[l] = bi_y[h];
-                        q[l] = ubody[c];
+                        q[l] = node_ub.empty() ? ubody[c] : node_ub[h];
                     }
                 double a[16];
                 bil_rows(x, y, a);

@@ -456,6 +456,14 @@ int orc_phase(void *h, const char *name) {
     return 0;
 }
 
+// per-point body velocities of a LUO case after its set-up (orc::Solver::set_body_velocity)
+int orc_set_body_velocity(void *h, const double *ub, const double *vb, int n, double omega) {
+    auto *S = (orc::Solver *)h;
+    if (n != S->B.n || S->P.solver_type != orc::LUO) return -1;
+    S->set_body_velocity(ub, vb, omega);
+    return 0;
+}
+
 long orc_get(void *h, const char *name, void *out, long n) {
     auto *S = (orc::Solver *)h;
     if (auto *f = S->field(name)) {

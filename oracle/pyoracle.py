@@ -91,6 +91,8 @@ def lib():
         L.orc_viv_update.restype = None
         L.orc_viv_update.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
                                      _DP, _DP]
+        L.orc_set_body_velocity.restype = C.c_int
+        L.orc_set_body_velocity.argtypes = [C.c_void_p, _DP, _DP, C.c_int, C.c_double]
         L.orc_embedded_stats.restype = None
         L.orc_embedded_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
         _lib = L
@@ -286,6 +288,13 @@ class Oracle:
         keys = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "fxx", "fxy", "fxu",
                 "startI", "startJ", "numCellsX", "numCellsY", "midX", "midY", "amg_levels", "sc_passes", "centerV"]
         return dict(zip(keys, out[:len(keys)]))
+
+    def set_body_velocity(self, ub, vb, omega=0.0):
+        """per-point body velocities and the solid nodes' rotation rate (LUO cases)"""
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        vb = np.ascontiguousarray(vb, dtype=np.float64)
+        if lib().orc_set_body_velocity(self.h, dptr(ub), dptr(vb), ub.size, float(omega)) != 0:
+            raise ValueError("per-point body velocities need a LUO case and one value per body point")
 
     def embedded_stats(self):
         """Slot anomalies of the embedded assembly summed over all steps: for LHS1 then

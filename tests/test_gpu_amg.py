@@ -3,8 +3,9 @@
 The small step tests reach two AMG levels at most; these cases reach three or four.
 Bars: one V-cycle with the FP64 hierarchy within 1e-12 relative L2 of the oracle's
 (the sub-wavefront row sums of the coarse operators add in a different order than
-the oracle's sequential sums); with FP32 matrix values within 1e-5 of FP64 (matrix
-rounding only, the vectors stay FP64); Luo steps on the 336x336 cylinder grid (stencil
+the oracle's sequential sums); the FP32 V-cycle within 1e-5 of FP64; BiCGSTAB with
+FP32 matrix values and FP64 vectors within one iteration of the oracle's count, and with
+the whole V-cycle in FP32 within 20% more; Luo steps on the 336x336 cylinder grid (stencil
 finest level, three levels) within 1e-10 for u and p and 1e-8 for drag and lift.
 """
 import numpy as np
@@ -65,9 +66,15 @@ def test_bicgstab_amg_matches_oracle():
     xo, ito, _ = o.bicgstab(b, np.zeros_like(b), precond="SMOOTHED_AGGREGATION", rtol=1e-10, maxit=500)
     assert itg == ito
     assert rel_l2(xg, xo) < 1e-9
+    # FP32 matrix values, FP64 vectors: the FP64 iteration count to within one
+    x, it, _, ok = g.bicgstab(b, precond="SMOOTHED_AGGREGATION", rtol=1e-10, maxit=500, amg_precision=3264)
+    assert ok and abs(it - ito) <= 1
+    assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b) * 1.0001
+    # the whole V-cycle in FP32 (CSR finest level, no deflation of the constant mode here):
+    # the same residual bar within 20% more iterations
     x32, it32, _, ok32 = g.bicgstab(b, precond="SMOOTHED_AGGREGATION", rtol=1e-10, maxit=500, amg_precision=32)
-    assert ok32
-    assert abs(it32 - ito) <= 1
+    print("iterations: oracle", ito, "fp32 values", it, "fp32 V-cycle", it32, flush=True)
+    assert ok32 and it32 <= 1.2 * ito + 1
     assert np.linalg.norm(b - A @ x32) <= 1e-10 * np.linalg.norm(b) * 1.0001
 
 

@@ -1,6 +1,7 @@
-"""Mixed-precision Poisson solve (-amgPrecision 32: the AMG hierarchy's matrix values in
-FP32, every vector and all arithmetic in FP64) against the FP64 oracle on the 336x336
-Luo cylinder with the stencil finest level. Bar: |dC_D| <= 1e-6 per step (the precision
+"""Mixed-precision Poisson solve (-amgPrecision 32: the AMG V-cycle in FP32, its matrix
+values, level vectors and smoother arithmetic, with the operator's pinned constant mode
+removed and restored in FP64 around it; the Krylov vectors, dots and the projection in
+FP64) against the FP64 oracle on the 336x336 Luo cylinder with the stencil finest level. Bar: |dC_D| <= 1e-6 per step (the precision
 form's accuracy target); u within 1e-8 relative L2, since both forms solve to the same
 residual tolerance and only the preconditioner differs."""
 import numpy as np
