@@ -13,13 +13,14 @@ Krylov dot products and the projection stay FP64. The same steps are also run wi
 FP64 hierarchy and with FP64 CSR operators everywhere, and the drag coefficients of the
 forms are compared.
 
-The discrete adjoint (``adjoint``): the gradient of the mean drag over steps 200..400
-of a 400-step run of the rotating cylinder at Re=100 (case
-``cuibm_amd/cases/rotatingCylinderRe100_luo``: 498x498 stretched grid with h=0.01 near
-the body, 256 body points with tangential velocity alpha (D/2) omega_ref, alpha=0.5,
-omega_ref=2, plus seeded N(0, 0.01^2) normal blowing, dt=0.01, solves to 1e-12) with
-respect to alpha and the 256 blowing values, checkpoints every 20 steps; seconds per
-gradient = the forward run plus the backward sweep.
+The discrete adjoint (``adjoint``): the gradient of the mean drag over t = 2..4 of the
+rotating cylinder at Re=100 (case ``cuibm_amd/cases/rotatingCylinderRe100_luo``: 498x498
+stretched grid with h=0.01 near the body, 256 body points with tangential velocity
+alpha (D/2) omega_ref, alpha=0.5, omega_ref=2, plus seeded N(0, 0.01^2) normal blowing,
+solves to 1e-12) with respect to alpha and the 256 blowing values, checkpoints every 20
+steps; seconds per gradient = the forward run plus the backward sweep. At dt=0.01 this
+grid puts the convective CFL number at 2.2 and the start diverges within 50 steps; the
+case runs dt=0.005 (CFL 1.1), 800 steps, mean over steps 400..800.
 
 A step is one full ``stepTime`` (rhs1, velocity solve, rhs2, Poisson solve, IBM
 interpolation/blending, projection, control-volume force) on fields resident in
@@ -260,7 +261,7 @@ def adjoint_bench(cu):
     import numpy as np
     from cuibm_amd.adjoint import RotatingCylinder
 
-    p = RotatingCylinder(os.path.join(cu.CASES, "rotatingCylinderRe100_luo"), nsteps=400, n0=200, n1=400, every=20)
+    p = RotatingCylinder(os.path.join(cu.CASES, "rotatingCylinderRe100_luo"), nsteps=800, n0=400, n1=800, every=20)
     blow = np.random.default_rng(2024).normal(0.0, 0.01, p.npts)
     t0 = time.perf_counter()
     J, ga, gb = p.evaluate(0.5, blow)
@@ -279,9 +280,9 @@ def adjoint_bench(cu):
         "J": J,
         "dJ_dalpha": ga,
         "dJ_dblow_norm": float(np.linalg.norm(gb)),
-        "workload": "rotatingCylinderRe100_luo: 498x498, 256 points, alpha=0.5 + seeded blowing, 400 steps of "
-                    "dt=0.01, J = mean C_D over steps 200..400, gradient w.r.t. alpha and 256 blowing values, "
-                    "checkpoints every 20 steps, FP64, solves to 1e-12",
+        "workload": "rotatingCylinderRe100_luo: 498x498, 256 points, alpha=0.5 + seeded blowing, 800 steps of "
+                    "dt=0.005, J = mean C_D over steps 400..800 (t = 2..4), gradient w.r.t. alpha and 256 blowing "
+                    "values, checkpoints every 20 steps, FP64, solves to 1e-12",
         "timing": "wall clock of one call (fresh solver set-up, forward run, backward sweep)",
     }
 
@@ -404,7 +405,7 @@ def main():
         log(f"[bench] fp64 csr: {csr['steps_per_s']:.2f} steps/s, Poisson its/step {csr['poisson_iters_per_step']:.1f}")
     adj = None
     if not args.no_adjoint:
-        log("[bench] discrete adjoint of the rotating cylinder (498x498, 400 steps)")
+        log("[bench] discrete adjoint of the rotating cylinder (498x498, 800 steps)")
         adj = adjoint_bench(cu)
     cpu = None
     if args.cpu_steps > 0:

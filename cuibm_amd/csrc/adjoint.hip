@@ -15,7 +15,7 @@
 //   u2  = uh2 - dt Grad p1,  u' = S(G(u2)) + body terms
 //   Fx  = F(u', uold = u1, p1)
 // and the adjoint runs these backwards: transposed solves with LHS1^T (Jacobi) and LHS2^T
-// (smoothed aggregation built on LHS2^T), transposed interpolation stencils as gathers
+// (the forward V-cycle conjugated by the cell areas), transposed interpolation stencils as gathers
 // through inverse maps built once per tagging (no atomics), the transposed divergence and
 // gradient, and the force gradient. The transposed Jacobian of f and N is taken from the
 // forward kernel itself: both are quadratic in u1 and linear in xp', so the central
@@ -354,12 +354,56 @@ __global__ void k_force_grad(GridDev g, double nu, double dt, BoxArgs b, double 
     }
 }
 
+// t = r / w or t = r * w, elementwise; returns at once while the Krylov stop code is set
+__global__ void k_wscale(long n, const double *__restrict__ r, const double *__restrict__ w, double *__restrict__ t,
+                         int divide, const double *__restrict__ stop) {
+    if (stop && *stop != 0.0) return;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        t[i] = divide ? r[i] / w[i] : r[i] * w[i];
+}
+
+// A preconditioner for LHS2^T from the forward one: with W the cell areas, W LHS2 is
+// symmetric (each coupling is -dt dy_J / ((dx_I + dx_I+1) / 2) from either side, and the
+// pins are diagonal), so LHS2^T = W LHS2 W^-1 and W M W^-1 approximates LHS2^-T wherever M
+// approximates LHS2^-1. The forward V-cycle (stencil finest level, FP32 or FP64) then
+// serves the transposed solve, whose operator stays the transposed CSR matrix.
+class ConjugatedPrecond : public Preconditioner {
+  public:
+    ConjugatedPrecond(Preconditioner &M, const double *w, long n) : M_(M), w_(w), n_(n), t_(n), u_(n) {}
+    void apply(const double *r, double *z, hipStream_t s) override {
+        const int g = blocks_for(n_);
+        hipLaunchKernelGGL(k_wscale, dim3(g), dim3(BS), 0, s, n_, r, w_, t_.p(), 1, stop_);
+        CUIBM_LAUNCH_CHECK();
+        M_.apply(t_.p(), u_.p(), s);
+        hipLaunchKernelGGL(k_wscale, dim3(g), dim3(BS), 0, s, n_, u_.p(), w_, z, 0, stop_);
+        CUIBM_LAUNCH_CHECK();
+    }
+    void set_stop(const double *stop) override {
+        stop_ = stop;
+        M_.set_stop(stop);
+    }
+    std::string name() const override { return "W " + M_.name() + " W^-1"; }
+    double bytes_per_apply() const override { return M_.bytes_per_apply() + 2 * 24.0 * n_; }
+
+  private:
+    Preconditioner &M_;
+    const double *w_;
+    long n_;
+    DArray<double> t_, u_;
+};
+
+__global__ void k_cell_area(GridDev g, double *__restrict__ w) {
+    for (int ip = blockIdx.x * blockDim.x + threadIdx.x; ip < g.nx * g.ny; ip += gridDim.x * blockDim.x)
+        w[ip] = g.dx[ip % g.nx] * g.dy[ip / g.nx];
+}
+
 }  // namespace
 
 struct luoIBM::Adjoint {
     DCsr<double> L1T, L2T;
     std::unique_ptr<JacobiPrecond> M1;
-    std::unique_ptr<AmgPrecond> M2;
+    std::unique_ptr<ConjugatedPrecond> M2;
+    DArray<double> area;  // cell areas (W)
     std::unique_ptr<Bicgstab> k1, k2;
     std::unique_ptr<CsrOp> op1, op2;
     TMap gnT, hnT, hnpT, gnpT;           // transposed stencils
@@ -459,9 +503,10 @@ void luoIBM::adjointRun(int nsteps, int n0, int n1, int every, bool backward, do
         A.op1.reset(new CsrOp(A.L1T));
         A.op2.reset(new CsrOp(A.L2T));
         A.M1.reset(new JacobiPrecond(A.L1T, s));
-        AmgParams ap;
-        ap.precision = 64;
-        A.M2.reset(new AmgPrecond(h2, ap, s));
+        A.area.alloc(numP);
+        hipLaunchKernelGGL(k_cell_area, dim3(blocks_for(numP)), dim3(BS), 0, s, grid(), A.area.p());
+        CUIBM_LAUNCH_CHECK();
+        A.M2.reset(new ConjugatedPrecond(*PC2, A.area.p(), numP));
         A.k1.reset(new Bicgstab(numUV));
         A.k2.reset(new Bicgstab(numP));
         auto host_st = [&](const Stencils &st, std::vector<int> &node, std::vector<int> &idx, std::vector<double> &cf) {
@@ -535,7 +580,15 @@ void luoIBM::adjointRun(int nsteps, int n0, int n1, int every, bool backward, do
         tmap_gather(A.gnT, lm.p(), out, s);
     };
     double itA1 = 0, itA2 = 0;
-    const double rtolA = 1e-13;
+    // the adjoint solves stop at the forward solves' tolerances and limits
+    const double rtol1 = db->velocitySolve.tolerance, rtol2 = db->PoissonSolve.tolerance;
+    const int maxit1 = db->velocitySolve.maxIterations, maxit2 = db->PoissonSolve.maxIterations;
+    auto check = [&](const SolveStat &st, const char *what, int n) {
+        if (!st.converged)
+            throw Error(std::string("adjoint ") + what + " solve did not converge at time step " + std::to_string(n) +
+                            " (" + std::to_string(st.iterations) + " iterations)",
+                        ERR_NOT_CONVERGED);
+    };
     for (int seg = (nsteps - 1) / every; seg >= 0; seg--) {
         const int c0 = seg * every, c1 = std::min(c0 + every, nsteps);
         // recompute the segment from its checkpoint, recording each step's inputs
@@ -590,7 +643,8 @@ void luoIBM::adjointRun(int nsteps, int n0, int n1, int every, bool backward, do
             // p0 = LHS2^-1 rhs2
             zero(lr2, np);
             {
-                const SolveStat st = A.k2->solve(*A.op2, lr2.p(), lp0.p(), *A.M2, rtolA, 20000, s);
+                const SolveStat st = A.k2->solve(*A.op2, lr2.p(), lp0.p(), *A.M2, rtol2, maxit2, s);
+                check(st, "Poisson", n);
                 itA2 += st.iterations;
             }
             // rhs2 = D uh2 + d(xp'): lambda_uh2 += D^T lr2; lambda_xp'[J] -= lr2[east cell] / dx
@@ -607,7 +661,8 @@ void luoIBM::adjointRun(int nsteps, int n0, int n1, int every, bool backward, do
             // uh0 = LHS1^-1 rhs1
             zero(lr1, nuv);
             {
-                const SolveStat st = A.k1->solve(*A.op1, lr1.p(), luh.p(), *A.M1, rtolA, 20000, s);
+                const SolveStat st = A.k1->solve(*A.op1, lr1.p(), luh.p(), *A.M1, rtol1, maxit1, s);
+                check(st, "velocity", n);
                 itA1 += st.iterations;
             }
             // rhs1 = f(u1, Nold, xp'), N' = N(u1, xp'): transposed Jacobians by colour probes

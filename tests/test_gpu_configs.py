@@ -128,6 +128,33 @@ def test_cylinder_336_full_run_matches_oracle_series(method):
     g.close()
 
 
+def test_cylinder_1024_at_half_time_step_is_stable_and_matches_kl95_re550():
+    """The configured 1024x1024 Re = 550 case diverges from step ~54 at dt = 0.0025 (the
+    convective CFL number passes 1 at the shoulders; the oracle diverges identically). At
+    dt = 0.00125 it runs stably, and its drag follows Koumoutsakos & Leonard (1995) at
+    Re = 550 (their time * 0.5): within 2% for t = 0.03-0.23 (260 steps, mixed form)."""
+    with open(os.path.join(GOLDEN, "literature.json")) as f:
+        kl = json.load(f)["kl95_re550"]
+    g = cuibm_amd.Solver(case("cylinderRe550_1024_luo"), "-dt", "0.00125", diagnostics=False)
+    g.initialise()
+    n = 260
+    cd, umax = [], []
+    for k in range(1, n + 1):
+        g.step_time()
+        cd.append(2 * g.info()["forceX"])
+        if k % 20 == 0:
+            umax.append(float(np.max(np.abs(g.get("u")))))
+            print(f"step {k}: C_D {cd[-1]:.5f} max|u| {umax[-1]:.3f}", flush=True)
+    t = 0.00125 * np.arange(1, n + 1)
+    tk, ck = 0.5 * np.array(kl["t"]), np.array(kl["cd"])
+    m = (tk >= 0.03) & (tk <= 0.23)
+    ratio = np.interp(tk[m], t, cd) / ck[m]
+    print("C_D / KL95 (Re 550):", np.round(ratio, 4), flush=True)
+    assert m.sum() >= 6 and np.all(np.abs(ratio - 1) < 0.02)
+    assert np.all(np.isfinite(cd)) and max(umax) < 2.1 and umax[-1] < umax[0]
+    g.close()
+
+
 def test_ensemble_64_cases_against_cases_run_alone(tmp_path):
     """The 64-case draw (seed 2024: edges 96-512, Re 40-200, h = D/40, 500 steps, Luo
     external, FP32 Poisson V-cycle) run as one ensemble; every case's drag and lift
