@@ -287,29 +287,36 @@ def adjoint_bench(cu):
     }
 
 
-def ensemble_bench(cu, n, steps, workers, check):
+def ensemble_bench(cu, n, steps, workers, check, prec="3264"):
     """n seeded cylinder cases of 96^2..512^2 cells and Re 40..200 (cuibm_amd.ensemble),
-    run concurrently with the FP32 V-cycle; every (n // check)-th case is then
-    run alone with the FP64 hierarchy and its drag coefficient compared"""
+    run in lockstep (every phase one launch over all cases) and, for comparison, as
+    concurrent solvers on their own streams; every (n // check)-th case is then run alone
+    with the FP64 hierarchy and its drag coefficient compared with both"""
     import tempfile
     from cuibm_amd import ensemble as ens
 
     folder = tempfile.mkdtemp(prefix="cuibm_ensemble_")
     paths, cases = ens.write_cases(folder, n=n, steps=steps)
     cells = sum(c[0] ** 2 for c in cases)
-    log(f"[bench] ensemble: {n} cases, {cells} cells, {steps} steps each, {workers} host workers")
-    r = cu.ensemble_run(paths, "-amgPrecision", "32", steps=steps, workers=workers)
+    log(f"[bench] ensemble: {n} cases, {cells} cells, {steps} steps each, AMG precision {prec}")
+    r = cu.ensemble_run(paths, "-amgPrecision", prec, steps=steps, workers=workers, lockstep=True)
     rate = cells * steps / r["steps_wall_s"]
-    log(f"[bench] ensemble: {rate / 1e6:.1f} M cell-updates/s over the steps ({r['steps_wall_s']:.2f} s), "
+    log(f"[bench] lockstep: {rate / 1e6:.1f} M cell-updates/s over the steps ({r['steps_wall_s']:.2f} s), "
         f"{r['wall_s']:.2f} s with set-up")
+    rc = cu.ensemble_run(paths, "-amgPrecision", prec, steps=steps, workers=workers)
+    rate_c = cells * steps / rc["steps_wall_s"]
+    log(f"[bench] concurrent solvers ({workers} host threads): {rate_c / 1e6:.1f} M cell-updates/s "
+        f"({rc['steps_wall_s']:.2f} s)")
     picked = list(range(0, n, max(1, n // check)))[:check]
-    dcd, t_alone = [], 0.0
+    dcd, dcd_c, t_alone = [], [], 0.0
     for i in picked:
         s = cu.Solver(paths[i], "-amgPrecision", "64", diagnostics=False)
         s.initialise()
         ms = s.time_steps(steps)
         t_alone += ms * 1e-3
-        dcd.append(abs(2.0 * s.info()["forceX"] - r["cd"][i]))
+        cd64 = 2.0 * s.info()["forceX"]
+        dcd.append(abs(cd64 - r["cd"][i]))
+        dcd_c.append(abs(cd64 - rc["cd"][i]))
         s.close()
         log(f"[bench]   case {i} ({cases[i][0]}^2, Re {cases[i][1]:.1f}) alone in FP64: |dC_D| {dcd[-1]:.2e}")
     alone_rate = sum(cases[i][0] ** 2 for i in picked) * steps / t_alone
@@ -321,11 +328,16 @@ def ensemble_bench(cu, n, steps, workers, check):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n} impulsively started cylinders, edge 96..512 cells (seeded), Re 40..200, "
                                f"h=D/40 on [-0.6,0.6] stretched to [-10,10], dt=0.01, Luo external, Poisson FP64 "
-                               f"BiCGSTAB rtol 1e-8 + FP32 SA-AMG V-cycle, velocity BiCGSTAB+Jacobi rtol 1e-8",
-                   "cases": n, "steps": steps, "cells": cells, "workers": workers},
+                               f"BiCGSTAB rtol 1e-8 + SA-AMG V-cycle (precision {prec}), velocity BiCGSTAB+Jacobi "
+                               f"rtol 1e-8; lockstep: every phase one launch over all cases",
+                   "cases": n, "steps": steps, "cells": cells, "workers": workers, "amg_precision": prec},
         "steps_wall_s": round(r["steps_wall_s"], 3), "wall_s_with_setup": round(r["wall_s"], 3),
+        "concurrent_cell_updates_per_s": round(rate_c, 1),
+        "concurrent_steps_wall_s": round(rc["steps_wall_s"], 3),
+        "speedup_lockstep_vs_concurrent": round(rate / rate_c, 3),
         "alone_fp64_cell_updates_per_s": round(alone_rate, 1),
-        "check": {"cases": picked, "max_abs_dcd_vs_alone_fp64": max(dcd)},
+        "check": {"cases": picked, "max_abs_dcd_vs_alone_fp64": max(dcd),
+                  "max_abs_dcd_concurrent_vs_alone_fp64": max(dcd_c)},
     }
 
 
@@ -355,6 +367,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-workers", type=int, default=16)
     ap.add_argument("--ensemble-check", type=int, default=8, help="cases re-run alone in FP64 for the C_D check")
+    ap.add_argument("--ensemble-precision", default="3264", help="AMG precision of the ensemble cases (64, 32, 3264)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -368,7 +381,7 @@ def main():
     case = os.path.join(cu.CASES, CASE)
     if args.ensemble > 0:
         print(json.dumps(ensemble_bench(cu, args.ensemble, args.ensemble_steps, args.ensemble_workers,
-                                        args.ensemble_check)), flush=True)
+                                        args.ensemble_check, args.ensemble_precision)), flush=True)
         return
     if args.published:
         rows = published_runs(cu)

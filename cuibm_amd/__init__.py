@@ -105,6 +105,8 @@ def lib():
     L.cuibm_ensemble_run.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int,
                                      C.POINTER(C.c_longlong), _DP, _DP, _DP, _DP, C.c_char_p, C.c_int]
     L.cuibm_ensemble_run.restype = C.c_int
+    L.cuibm_ensemble_lockstep.argtypes = L.cuibm_ensemble_run.argtypes
+    L.cuibm_ensemble_lockstep.restype = C.c_int
     L.cuibm_launch_bytes.argtypes = [C.c_void_p, C.c_char_p, _DP, C.c_long]
     L.cuibm_launch_bytes.restype = C.c_long
     L.cuibm_set_body_velocity.argtypes = [C.c_void_p, _DP, _DP, C.c_int, C.c_double]
@@ -128,10 +130,13 @@ def _dp(a):
     return a.ctypes.data_as(_DP)
 
 
-def ensemble_run(case_folders, *args, steps=0, workers=8):
-    """Run independent cases to completion concurrently (cuibm_ensemble_run): each on its
-    own solver and HIP stream, driven by `workers` host threads; `args` are common
-    command-line overrides. Returns a dict with per-case cells, cd, cl and the wall times."""
+def ensemble_run(case_folders, *args, steps=0, workers=8, lockstep=False):
+    """Run independent cases to completion on one GPU; `args` are common command-line
+    overrides. Default (cuibm_ensemble_run): each case on its own solver and HIP stream,
+    driven by `workers` host threads. lockstep=True (cuibm_ensemble_lockstep): every phase
+    of the step is one launch over all cases, each case's solves stopping at its own
+    tolerance (`workers` threads run the set-up). Returns a dict with per-case cells, cd,
+    cl and the wall times."""
     n = len(case_folders)
     folders = (C.c_char_p * n)(*[os.fsencode(str(f)) for f in case_folders])
     argv = (C.c_char_p * max(len(args), 1))(*[os.fsencode(str(a)) for a in args])
@@ -139,7 +144,8 @@ def ensemble_run(case_folders, *args, steps=0, workers=8):
     cd, cl = np.zeros(n), np.zeros(n)
     wall, wall_steps = C.c_double(0), C.c_double(0)
     err = C.create_string_buffer(1024)
-    rc = lib().cuibm_ensemble_run(n, folders, len(args), argv, int(steps), int(workers),
+    fn = lib().cuibm_ensemble_lockstep if lockstep else lib().cuibm_ensemble_run
+    rc = fn(n, folders, len(args), argv, int(steps), int(workers),
                                   cells.ctypes.data_as(C.POINTER(C.c_longlong)), _dp(cd), _dp(cl), C.byref(wall),
                                   C.byref(wall_steps), err, len(err))
     if rc < 0:

@@ -556,7 +556,8 @@ void luoIBM::adjointRun(int nsteps, int n0, int n1, int every, bool backward, do
     DArray<double> lm(nuv), lu2(nuv), lp1(np), lq(np), lpst(np), lp0(np), lr2(np), luh(nuv), lst(nuv), lr1(nuv);
     DArray<double> lu1(nuv), lNo(nuv), lxpP(nxp);
     DArray<double> u1(nuv), up(nuv), um(nuv), Rp(nuv), Rm(nuv), Np(nuv), Nm(nuv), xpP(nxp), xpp(nxp), xpm(nxp);
-    for (auto *a : {&lu, &lN}) launch_fill(a->p(), 0.0, nuv, s);
+    for (auto *a : {&lu, &lN, &lr1}) launch_fill(a->p(), 0.0, nuv, s);
+    launch_fill(lr2.p(), 0.0, np, s);
     launch_fill(lxp.p(), 0.0, nxp, s);
     // per-step records of a segment: u_in, N_in, xp_in; and the state after the segment
     DArray<double> rec((size_t)(every + 1) * (2 * nuv + nxp));
@@ -641,7 +642,8 @@ void luoIBM::adjointRun(int nsteps, int n0, int n1, int every, bool backward, do
                                    lq.p(), lp0.p(), lpst.p());
             tmap_gather(A.hnpT, lpst.p(), lp0.p(), s);
             // p0 = LHS2^-1 rhs2
-            zero(lr2, np);
+            // warm start: the previous (later) step's adjoint pressure, as the forward
+            // solve starts from the previous pressure
             {
                 const SolveStat st = A.k2->solve(*A.op2, lr2.p(), lp0.p(), *A.M2, rtol2, maxit2, s);
                 check(st, "Poisson", n);
@@ -659,7 +661,7 @@ void luoIBM::adjointRun(int nsteps, int n0, int n1, int every, bool backward, do
                 hipLaunchKernelGGL(k_blend_T, dim3(nblk(hnV.n)), dim3(BS), 0, s, hnV.n, hnV.node.p(), hnV.extra.p(),
                                    lm.p(), luh.p(), lst.p());
             // uh0 = LHS1^-1 rhs1
-            zero(lr1, nuv);
+            // warm start likewise from the later step's adjoint velocity
             {
                 const SolveStat st = A.k1->solve(*A.op1, lr1.p(), luh.p(), *A.M1, rtol1, maxit1, s);
                 check(st, "velocity", n);

@@ -14,9 +14,11 @@
 // reads the FP64 Krylov vector and writes its result in FP64 but keeps every level
 // vector, smoother sweep and product in FP32 (the coarsest solve accumulates in FP64).
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <type_traits>
 
+#include "batch.hpp"
 #include "krylov.hpp"
 #include "stencil.hpp"
 
@@ -208,12 +210,10 @@ __device__ __forceinline__ void load4<double>(const double *p, double (&v)[4]) {
 }
 
 template <class V, class X>
-__global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restrict__ Ainv, const X *__restrict__ b,
-                                               X *__restrict__ x, const double *__restrict__ stop) {
-    const double stopv = stop ? *stop : 0.0;
+__device__ __forceinline__ void dense_row(int n, int ld, const V *__restrict__ Ainv, const X *__restrict__ b,
+                                          X *__restrict__ x, int row, double stopv) {
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;  // no barrier in this kernel
+    if (row >= n) return;  // no barrier in this function
     const V *a = Ainv + (size_t)row * ld;
     const int n4 = n & ~3;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -237,6 +237,29 @@ __global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restric
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if (lane == 0) x[row] = (X)s;
+}
+
+template <class V, class X>
+__global__ void __launch_bounds__(256) k_dense(int n, int ld, const V *__restrict__ Ainv, const X *__restrict__ b,
+                                               X *__restrict__ x, const double *__restrict__ stop) {
+    const double stopv = stop ? *stop : 0.0;
+    dense_row<V, X>(n, ld, Ainv, b, x, blockIdx.x * 4 + (threadIdx.x >> 6), stopv);
+}
+
+// the dense coarsest solves of several cases in one launch (lockstep batches): case c's
+// inverse, its size and row stride, and its offset in the level vectors
+template <class V>
+struct DenseCase {
+    const V *inv;
+    int n, ld;
+    long off;
+};
+
+template <class V, class X>
+__global__ void __launch_bounds__(256) k_b_dense(const DenseCase<V> *__restrict__ tab, BMapDev m,
+                                                 const X *__restrict__ b, X *__restrict__ x) {
+    const DenseCase<V> &D = tab[m.cs[blockIdx.x]];
+    dense_row<V, X>(D.n, D.ld, D.inv, b + D.off, x + D.off, m.loc[blockIdx.x] * 4 + (threadIdx.x >> 6), 0.0);
 }
 
 }  // namespace
@@ -311,13 +334,60 @@ struct AmgDev {
         }
         nc = h.back().A.n;
         if (nc > 6144) throw Error("AMG coarsening stalled at " + std::to_string(nc) + " unknowns", ERR_GENERIC);
-        const std::vector<double> inv = dense_inverse(h.back().A);
+        std::vector<double> inv = dense_inverse(h.back().A);
+        if (fine.defl) deflate_coarse(inv, nc);
         ld = (nc + 3) & ~3;  // row stride, zero-padded to a multiple of 4 values
         std::vector<V> invt((size_t)nc * ld, V(0));
         for (int i = 0; i < nc; i++)
             for (int j = 0; j < nc; j++) invt[(size_t)i * ld + j] = (V)inv[(size_t)i * nc + j];
         coarse_inv.upload(invt, s);
         CUIBM_HIP(hipStreamSynchronize(s));
+    }
+
+    // With the finest level deflated (the FP32 V-cycle), the coarsest operator is still
+    // weakly pinned: its inverse has one eigenvalue 1/mu far above the others, whose
+    // eigenvector is close to the constant, and an FP32 copy of it (and FP32 level vectors
+    // holding its huge component) would drown everything else in rounding. The constant
+    // mode is already carried in FP64 by the finest-level deflation, so that component is
+    // removed here (Wielandt deflation, in FP64, before the FP32 copy):
+    //   inv <- inv - sigma v u^T / (u^T v),
+    // with sigma, v, u the dominant eigenvalue and right / left eigenvectors of inv by
+    // power iteration (the gap to the next eigenvalue makes a few iterations exact).
+    static void deflate_coarse(std::vector<double> &inv, int n) {
+        auto power = [&](bool left) {
+            std::vector<double> x(n, 1.0), y(n);
+            for (int it = 0; it < 60; it++) {
+                for (int i = 0; i < n; i++) {
+                    double a = 0;
+                    for (int j = 0; j < n; j++) a += (left ? inv[(size_t)j * n + i] : inv[(size_t)i * n + j]) * x[j];
+                    y[i] = a;
+                }
+                double nrm = 0;
+                for (double v : y) nrm += v * v;
+                nrm = std::sqrt(nrm);
+                if (!(nrm > 0)) break;
+                double diff = 0;
+                for (int i = 0; i < n; i++) {
+                    const double v = y[i] / nrm;
+                    diff = std::max(diff, std::fabs(v - x[i]));
+                    x[i] = v;
+                }
+                if (diff < 1e-15) break;
+            }
+            return x;
+        };
+        const std::vector<double> v = power(false), u = power(true);
+        double uv = 0, uiv = 0;
+        for (int i = 0; i < n; i++) {
+            uv += u[i] * v[i];
+            double a = 0;
+            for (int j = 0; j < n; j++) a += inv[(size_t)i * n + j] * v[j];
+            uiv += u[i] * a;
+        }
+        if (!(std::fabs(uv) > 1e-12)) return;  // no separated mode to remove
+        const double sigma = uiv / uv;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) inv[(size_t)i * n + j] -= sigma * v[i] * u[j] / uv;
     }
 
     // the preconditioner: one V-cycle from x = 0 on the FP64 Krylov vectors b and x
@@ -486,6 +556,8 @@ AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P
 
 AmgPrecond::~AmgPrecond() = default;
 
+double AmgPrecond::deflation_inv_wa() const { return fine_dev_.defl ? im_->d32.inv_wa : 0.0; }
+
 void AmgPrecond::set_stop(const double *stop) {
     stop_ = stop;
     im_->each([&](auto &d) {
@@ -634,5 +706,277 @@ double AmgPrecond::bytes_per_apply() const {
     for (double b : launch_bytes()) bytes += b;
     return bytes;
 }
+
+// ---- lockstep batches: the V-cycles of many cases as one preconditioner ----
+//
+// Cases are ordered by hierarchy depth (deepest first), so the cases present on level l
+// are a prefix of the case list, and within it the cases that still smooth on level l
+// (depth > l + 1) come before those whose coarsest (dense) level is l. Every case segment
+// is padded to a multiple of BATCH_PAD rows (empty CSR rows, zero D^-1), so the
+// concatenated block-diagonal CSR operators run through the single-case kernels and
+// each dense solve reads aligned vectors.
+
+namespace {
+
+template <class V>
+struct HCat {
+    int n = 0;
+    std::vector<int> rp{0}, ci;
+    std::vector<V> v;
+    // append the rows of src (columns mapped by col), then empty rows up to `rows`
+    template <class F>
+    void append(const std::vector<int> &srp, const std::vector<int> &sci, const std::vector<V> &sv, int rows, F col) {
+        const int sn = (int)srp.size() - 1;
+        for (int i = 0; i < sn; i++) {
+            for (int k = srp[i]; k < srp[i + 1]; k++) {
+                ci.push_back(col(sci[k]));
+                v.push_back(sv[k]);
+            }
+            rp.push_back((int)ci.size());
+        }
+        for (int i = sn; i < rows; i++) rp.push_back((int)ci.size());
+        n += rows;
+    }
+    void upload(DCsr<V> &d, int ncols, hipStream_t s) const {
+        // every row pointer and column inside the operator's shape (host check before any launch)
+        if ((int)rp.size() != n + 1 || rp.back() != (int)ci.size())
+            throw Error("batched V-cycle: inconsistent concatenated row pointers", ERR_GENERIC);
+        for (int i = 0; i < n; i++)
+            if (rp[i + 1] < rp[i]) throw Error("batched V-cycle: decreasing row pointers", ERR_GENERIC);
+        for (int j : ci)
+            if (j < 0 || j >= ncols) throw Error("batched V-cycle: a column outside the operator", ERR_GENERIC);
+        d.n = n;
+        d.ncols = ncols;
+        d.nnz = (int)ci.size();
+        d.rp.upload(rp, s);
+        d.ci.upload(ci.empty() ? std::vector<int>{0} : ci, s);
+        d.v.upload(v.empty() ? std::vector<V>{V(0)} : v, s);
+    }
+    int lanes() const {
+        HCsr h;
+        h.n = n;
+        h.ci.assign(ci.size(), 0);
+        return lanes_for(h);
+    }
+};
+
+}  // namespace
+
+template <class V, class X>
+struct BatchDev {
+    struct Lvl {
+        int n = 0, nS = 0;  // padded rows of the present cases, and of the smoothing prefix
+        DCsr<V> A, G, P, R;
+        int lanesA = 1, lanesG = 1, lanesP = 1, lanesR = 1;
+        DArray<V> dinv;
+        DArray<X> b, x, xt, r;
+        DArray<DenseCase<V>> dense;
+        BMap dmap;
+    };
+    std::vector<Lvl> lv;  // lv[0]: the finest level (P0, R0 and its two vectors)
+    int ncases = 0;
+    const BCase *tab = nullptr;
+    BMapDev pmap{}, tiles{};
+    const int *pfirst = nullptr;
+    BMap tilemap;
+    DArray<double> part, inv_wa;
+    double *defl = nullptr;
+    bool deflate = false;
+    double omega = 0.8;
+
+    void build(const std::vector<AmgDev<V, X> *> &cd, const std::vector<long> &offP, hipStream_t s) {
+        ncases = (int)cd.size();
+        int depth = 0;
+        for (auto *d : cd) depth = std::max(depth, (int)d->lv.size());
+        for (auto *d : cd)
+            if (d->lv.size() < 2 || !d->use_fine) throw Error("batched V-cycle: every case needs a stencil finest level and a coarser one", ERR_UNSUPPORTED);
+        for (int c = 1; c < ncases; c++)
+            if (cd[c]->lv.size() > cd[c - 1]->lv.size()) throw Error("batched V-cycle: cases must come deepest first", ERR_ARGUMENT);
+        lv.resize(depth);
+        // segment offsets per level: level 0 uses the pressure offsets
+        std::vector<std::vector<long>> off(depth, std::vector<long>(ncases, -1));
+        for (int c = 0; c < ncases; c++) off[0][c] = offP[c];
+        lv[0].n = (int)(offP.back() + batch_pad(cd.back()->lv[0].n));
+        for (int l = 1; l < depth; l++) {
+            long o = 0;
+            for (int c = 0; c < ncases; c++)
+                if ((int)cd[c]->lv.size() > l) {
+                    off[l][c] = o;
+                    o += batch_pad(cd[c]->lv[l].n);
+                }
+            lv[l].n = (int)o;
+        }
+        for (int l = 0; l < depth; l++) {
+            Lvl &L = lv[l];
+            HCat<V> A, G, P, R;
+            std::vector<V> dinv;
+            std::vector<DenseCase<V>> dense;
+            std::vector<long> dcount;
+            std::vector<int> dcase;
+            int nS = 0;
+            for (int c = 0; c < ncases; c++) {
+                AmgDev<V, X> &d = *cd[c];
+                if ((int)d.lv.size() <= l) continue;
+                const int rows = (int)batch_pad(d.lv[l].n);
+                if ((int)d.lv.size() - 1 == l) {  // this case's coarsest level
+                    dense.push_back(DenseCase<V>{d.coarse_inv.p(), d.nc, d.ld, off[l][c]});
+                    dcount.push_back(d.nc);
+                    continue;
+                }
+                nS = (int)(off[l][c] + rows);
+                AmgLevelDev<V, X> &S = d.lv[l];
+                const long o = off[l][c], oc = off[l + 1][c];
+                if (l > 0) {
+                    if (S.G.n == 0) throw Error("batched V-cycle: a CSR level without its fused restriction", ERR_UNSUPPORTED);
+                    A.append(S.A.rp.to_host(s), S.A.ci.to_host(s), S.A.v.to_host(s), rows, [&](int j) { return (int)(o + j); });
+                    std::vector<V> dv = S.dinv.to_host(s);
+                    dv.resize(rows, V(0));
+                    dinv.insert(dinv.end(), dv.begin(), dv.end());
+                }
+                const int crow = (int)batch_pad(d.lv[l + 1].n);
+                P.append(S.P.rp.to_host(s), S.P.ci.to_host(s), S.P.v.to_host(s), rows, [&](int j) { return (int)(oc + j); });
+                R.append(S.R.rp.to_host(s), S.R.ci.to_host(s), S.R.v.to_host(s), crow, [&](int j) { return (int)(o + j); });
+                if (l > 0) {
+                    const int nf = S.n;
+                    G.append(S.G.rp.to_host(s), S.G.ci.to_host(s), S.G.v.to_host(s), crow, [&, nf](int j) {
+                        return j < nf ? (int)(o + j) : (int)(-1 - (o + j - nf));  // x part: marked, shifted below
+                    });
+                }
+            }
+            L.nS = nS;
+            // the x part of G's columns goes after the nS b columns
+            for (int &j : G.ci)
+                if (j < 0) j = nS + (-1 - j);
+            if (nS > 0) {
+                if (l > 0) {
+                    A.upload(L.A, L.n, s);
+                    L.lanesA = A.lanes();
+                    G.upload(L.G, 2 * nS, s);
+                    L.lanesG = G.lanes();
+                    L.dinv.upload(dinv, s);
+                }
+                P.upload(L.P, lv[l + 1].n, s);
+                L.lanesP = P.lanes();
+                R.upload(L.R, nS, s);
+                L.lanesR = R.lanes();
+            }
+            if (!dense.empty()) {
+                L.dense.upload(dense, s);
+                L.dmap.build(dcount, 4, s);
+            }
+            if (l > 0) {
+                for (auto *a : {&L.b, &L.x, &L.xt, &L.r}) {
+                    a->alloc(std::max(L.n, 1));
+                    launch_fill_x(a->p(), L.n, s);
+                }
+            } else {
+                for (auto *a : {&L.xt, &L.r}) {
+                    a->alloc(std::max(L.n, 1));
+                    launch_fill_x(a->p(), L.n, s);
+                }
+            }
+        }
+        CUIBM_HIP(hipStreamSynchronize(s));
+    }
+
+    static void launch_fill_x(float *p, long n, hipStream_t s) { if (n) launch_fill_f(p, 0.0f, n, s); }
+    static void launch_fill_x(double *p, long n, hipStream_t s) { if (n) launch_fill(p, 0.0, n, s); }
+
+    void vcycle(size_t l, hipStream_t s) {
+        Lvl &L = lv[l];
+        const X w = (X)omega;
+        if (L.nS > 0) {
+            Lvl &C = lv[l + 1];
+            launch_pre2<V, X>(L.lanesA, L.A, L.dinv.p(), L.b.p(), L.xt.p(), w, s, nullptr);
+            launch_rres<V, X>(L.lanesG, L.G, L.nS, L.xt.p(), L.b.p(), C.b.p(), s, nullptr);
+            vcycle(l + 1, s);
+            launch_csr_op<V, X, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), X(0), s);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), L.b.p(), L.dinv.p(), L.r.p(), w, s);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.r.p(), L.b.p(), L.dinv.p(), L.x.p(), w, s);
+        }
+        if (L.dmap.nblk() > 0) {
+            hipLaunchKernelGGL((k_b_dense<V, X>), dim3(L.dmap.nblk()), dim3(256), 0, s, L.dense.p(), L.dmap.dev(),
+                               L.b.p(), L.x.p());
+            CUIBM_LAUNCH_CHECK();
+        }
+    }
+
+    void apply(const double *b, double *x, hipStream_t s) {
+        Lvl &F = lv[0], &C = lv[1];
+        const X w = (X)omega;
+        if (deflate) {
+            launch_b_wdot(tab, pmap, b, part.p(), s);
+            launch_b_wdot_finish(tab, ncases, pfirst, part.p(), inv_wa.p(), defl, s);
+        }
+        launch_b_p5_pre(tab, tiles, w, b, F.xt.p(), F.r.p(), s);
+        launch_csr_op<V, X, OP_SPMV>(F.lanesR, F.R, F.r.p(), nullptr, nullptr, C.b.p(), X(0), s);
+        vcycle(1, s);
+        launch_csr_op<V, X, OP_PROLONG>(F.lanesP, F.P, C.x.p(), nullptr, nullptr, F.xt.p(), X(0), s);
+        launch_b_p5_sweep2(tab, tiles, w, F.xt.p(), b, x, s);
+    }
+};
+
+struct BatchAmg::Impl {
+    int precision = 64;
+    BatchDev<double, double> d64;
+    BatchDev<float, float> d32;
+};
+
+BatchAmg::BatchAmg(const std::vector<AmgPrecond *> &amg, const std::vector<long> &offP, const BCase *tab_dev,
+                   const std::vector<BCase> &tab_host, const BMapDev &pmap, const int *pfirst, hipStream_t s)
+    : im_(new Impl) {
+    if (amg.empty()) throw Error("batched V-cycle without cases", ERR_ARGUMENT);
+    const int prec = amg[0]->precision();
+    for (auto *a : amg) {
+        if (a->precision() != prec) throw Error("batched V-cycle: cases of different AMG precisions", ERR_ARGUMENT);
+        if (!a->has_stencil_level()) throw Error("batched V-cycle: a case without its stencil finest level", ERR_UNSUPPORTED);
+    }
+    if (prec != 64 && prec != 32) throw Error("batched V-cycle: AMG precision 64 or 32", ERR_UNSUPPORTED);
+    im_->precision = prec;
+    const int n = (int)amg.size();
+    std::vector<long> tcount(n);
+    for (int c = 0; c < n; c++) tcount[c] = p5_tiles(tab_host[c].p5.nx, tab_host[c].p5.ny);
+    const bool deflate = tab_host[0].p5.defl != nullptr;
+    std::vector<double> iw(n, 0.0);
+    for (int c = 0; c < n; c++) {
+        if ((tab_host[c].p5.defl != nullptr) != deflate)
+            throw Error("batched V-cycle: deflation on some cases only", ERR_ARGUMENT);
+        iw[c] = amg[c]->deflation_inv_wa();
+    }
+    auto setup = [&](auto &B, const auto &cd) {
+        B.omega = amg[0]->prm_.omega;
+        B.tab = tab_dev;
+        B.pmap = pmap;
+        B.pfirst = pfirst;
+        B.build(cd, offP, s);
+        B.tilemap.build(tcount, 1, s);
+        B.tiles = B.tilemap.dev();
+        B.deflate = deflate;
+        if (deflate) {
+            B.inv_wa.upload(iw, s);
+            B.part.alloc(std::max(pmap.nblk, 1));
+            B.defl = const_cast<double *>(tab_host[0].p5.defl);  // the batch's array: case c at defl + c
+        }
+    };
+    if (prec == 32) {
+        std::vector<AmgDev<float, float> *> cd;
+        for (auto *a : amg) cd.push_back(&a->im_->d32);
+        setup(im_->d32, cd);
+    } else {
+        std::vector<AmgDev<double, double> *> cd;
+        for (auto *a : amg) cd.push_back(&a->im_->d64);
+        setup(im_->d64, cd);
+    }
+    CUIBM_HIP(hipStreamSynchronize(s));
+}
+
+BatchAmg::~BatchAmg() = default;
+
+void BatchAmg::apply(const double *r, double *z, hipStream_t s) {
+    if (im_->precision == 32) im_->d32.apply(r, z, s);
+    else im_->d64.apply(r, z, s);
+}
+
+int BatchAmg::levels() const { return im_->precision == 32 ? (int)im_->d32.lv.size() : (int)im_->d64.lv.size(); }
 
 }  // namespace cuibm

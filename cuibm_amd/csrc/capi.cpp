@@ -1,12 +1,16 @@
 // extern "C" boundary (include/cuibm_amd.h). Exceptions never cross it: every entry
 // point catches cuibm::Error / std::exception and returns a negative code.
+#include <chrono>
 #include <cstring>
+#include <mutex>
+#include <thread>
 #include <sstream>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/cuibm_amd.h"
+#include "batch.hpp"
 #include "noalloc.hpp"
 #include "solver.hpp"
 #include "yaml_lite.hpp"
@@ -437,3 +441,82 @@ int cuibm_amg_vcycle(cuibm_csr *A, int amg_precision, const double *b, double *x
 }
 
 }  // extern "C"
+
+extern "C" int cuibm_ensemble_lockstep(int n, const char *const *case_folders, int argc, const char *const *argv,
+                                       int steps, int workers, long long *cells, double *cd, double *cl,
+                                       double *wall_s, double *steps_wall_s, char *err, int err_len) {
+    auto now = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+    std::vector<cuibm_solver *> hs;
+    auto fail = [&](int rc, const std::string &m) {
+        if (err && err_len > 0) {
+            const size_t k = std::min((size_t)err_len - 1, m.size());
+            std::copy(m.begin(), m.begin() + k, err);
+            err[k] = '\0';
+        }
+        for (auto *h : hs) cuibm_destroy(h);
+        return rc;
+    };
+    if (n <= 0 || !case_folders) return fail(CUIBM_ERR_ARGUMENT, "no cases");
+    if (workers < 1 || workers > 64) return fail(CUIBM_ERR_ARGUMENT, "workers must be 1..64");
+    const double t0 = now();
+    int nsteps = -1;
+    for (int i = 0; i < n; i++) {
+        cuibm_solver *h = nullptr;
+        const int rc = cuibm_create(case_folders[i], argc, argv, &h);
+        if (rc != 0) return fail(rc, "case " + std::to_string(i) + ": " + cuibm_last_error());
+        hs.push_back(h);
+        const int k = steps > 0 ? steps : h->db.nt;
+        if (nsteps >= 0 && k != nsteps) return fail(CUIBM_ERR_ARGUMENT, "lockstep cases need equal step counts");
+        nsteps = k;
+    }
+    // set-up of every case on `workers` threads (tags, operators, AMG hierarchies)
+    std::mutex mu;
+    std::string msg;
+    int rc = 0, next = 0;
+    std::vector<std::thread> th;
+    for (int w = 0; w < std::min(workers, n); w++)
+        th.emplace_back([&] {
+            for (;;) {
+                int i;
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (next >= n || rc != 0) return;
+                    i = next++;
+                }
+                const int r = cuibm_initialise(hs[i]);
+                if (r != 0) {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (rc == 0) {
+                        rc = r;
+                        msg = "case " + std::to_string(i) + ": " + cuibm_last_error();
+                    }
+                    return;
+                }
+            }
+        });
+    for (auto &t : th) t.join();
+    if (rc != 0) return fail(rc, msg);
+    double t1 = 0, t2 = 0;
+    const int r2 = guard([&] {
+        for (auto *h : hs) CUIBM_HIP(hipStreamSynchronize(h->s->stream()));
+        std::vector<NavierStokesSolver *> ss;
+        for (auto *h : hs) ss.push_back(h->s.get());
+        LockstepBatch batch(ss, hs[0]->s->stream());
+        t1 = now();
+        for (int k = 0; k < nsteps; k++) batch.step();
+        std::vector<double> vcd, vcl;
+        batch.coefficients(vcd, vcl);  // synchronises
+        t2 = now();
+        for (int i = 0; i < n; i++) {
+            if (cells) cells[i] = (long long)hs[i]->s->info_cells();
+            if (cd) cd[i] = vcd[i];
+            if (cl) cl[i] = vcl[i];
+        }
+        return 0;
+    });
+    if (r2 != 0) return fail(r2, cuibm_last_error());
+    if (wall_s) *wall_s = t2 - t0;
+    if (steps_wall_s) *steps_wall_s = t2 - t1;
+    for (auto *h : hs) cuibm_destroy(h);
+    return 0;
+}

@@ -5,6 +5,7 @@
 // LDS across the 4 wavefronts of a 256-thread block) and writes one partial; a
 // single-block kernel then sums the partials in a fixed order. The result depends
 // only on n, never on scheduling.
+#include "batch.hpp"
 #include "krylov.hpp"
 #include "kernels.hpp"
 
@@ -257,34 +258,9 @@ void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const d
     CUIBM_LAUNCH_CHECK();
 }
 
-// sum the partial groups and update the recurrence scalars (one block); FIN_SS and
-// FIN_BETA then apply the host's stopping tests (cusp::monitor on ||s|| and ||r||, the
-// breakdown tests, the iteration limit) and record the outcome in SC_STOP
-__global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__ part, int op, double *__restrict__ sc,
-                                                      double tol, int maxit) {
-    const double stopv = (op != FIN_INIT && op != FIN_DOT2) ? sc[SC_STOP] : 0.0;  // tested after the loads
-    __shared__ double sh[2][16];
-    double v[2] = {0, 0};
-    const int ng = (op == FIN_ALPHA || op == FIN_SS) ? 1 : 2;
-    for (int g = 0; g < ng; g++) {
-        double t = 0;
-        for (int k = threadIdx.x; k < RED_GRID; k += blockDim.x) t += part[g * RED_GRID + k];
-        v[g] = t;
-    }
-    if (stopv != 0.0) return;  // uniform across the block, before the barrier
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int g = 0; g < ng; g++) {
-        double t = wave_sum(v[g]);
-        if (lane == 0) sh[g][wid] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double tot[2] = {0, 0};
-    for (int g = 0; g < ng; g++) {
-        double t = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[g][w];
-        tot[g] = t;
-    }
+// the scalar updates and stopping tests of one finish, from the summed partials tot
+__device__ __forceinline__ void finish_update(int op, const double (&tot)[2], double *__restrict__ sc, double tol,
+                                              int maxit) {
     switch (op) {
         case FIN_INIT:  // (r.r, b.b); a (re)start clears the stop code, the count goes on
             sc[SC_RR] = tot[0];
@@ -328,6 +304,37 @@ __global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__
             sc[SC_D1] = tot[1];
             break;
     }
+}
+
+// sum the partial groups and update the recurrence scalars (one block); FIN_SS and
+// FIN_BETA then apply the host's stopping tests (cusp::monitor on ||s|| and ||r||, the
+// breakdown tests, the iteration limit) and record the outcome in SC_STOP
+__global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__ part, int op, double *__restrict__ sc,
+                                                      double tol, int maxit) {
+    const double stopv = (op != FIN_INIT && op != FIN_DOT2) ? sc[SC_STOP] : 0.0;  // tested after the loads
+    __shared__ double sh[2][16];
+    double v[2] = {0, 0};
+    const int ng = (op == FIN_ALPHA || op == FIN_SS) ? 1 : 2;
+    for (int g = 0; g < ng; g++) {
+        double t = 0;
+        for (int k = threadIdx.x; k < RED_GRID; k += blockDim.x) t += part[g * RED_GRID + k];
+        v[g] = t;
+    }
+    if (stopv != 0.0) return;  // uniform across the block, before the barrier
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int g = 0; g < ng; g++) {
+        double t = wave_sum(v[g]);
+        if (lane == 0) sh[g][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double tot[2] = {0, 0};
+    for (int g = 0; g < ng; g++) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[g][w];
+        tot[g] = t;
+    }
+    finish_update(op, tot, sc, tol, maxit);
 }
 
 void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s, double tol, int maxit) {
@@ -384,6 +391,137 @@ __global__ void k_diag_inv(int n, const int *__restrict__ rp, const int *__restr
 
 void launch_diag_inv(const DCsr<double> &A, double *dinv, hipStream_t s) {
     hipLaunchKernelGGL(k_diag_inv, dim3(grid_for(A.n)), dim3(BS), 0, s, A.n, A.rp.p(), A.ci.p(), A.v.p(), dinv);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// ---- lockstep batches: BiCGSTAB vector updates with per-case scalars ----
+// Each block belongs to one case, so a stopped case's blocks return before any barrier.
+
+struct BSeg {
+    double *sc;
+    long off;
+    int n;
+};
+__device__ __forceinline__ BSeg bseg(int op, const BCase &C) {
+    return op == BOP_VEL ? BSeg{C.scV, C.offUV, C.numUV} : BSeg{C.scP, C.offP, C.numP};
+}
+
+__global__ void __launch_bounds__(BS) k_b_bicg_s(int op, const BCase *__restrict__ tab, BMapDev m,
+                                                 const double *__restrict__ r, const double *__restrict__ AMp,
+                                                 double *__restrict__ s, double *__restrict__ part) {
+    const BSeg g = bseg(op, tab[m.cs[blockIdx.x]]);
+    if (g.sc[SC_STOP] != 0.0) return;
+    const double alpha = g.sc[SC_ALPHA];
+    const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
+    double acc[1] = {0};
+    if (i < g.n) {
+        const double si = r[g.off + i] - alpha * AMp[g.off + i];
+        s[g.off + i] = si;
+        acc[0] = si * si;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+__global__ void __launch_bounds__(BS) k_b_bicg_xr(int op, const BCase *__restrict__ tab, BMapDev m,
+                                                  double *__restrict__ x, const double *__restrict__ Mp,
+                                                  const double *__restrict__ Ms, double *__restrict__ r,
+                                                  const double *__restrict__ s, const double *__restrict__ AMs,
+                                                  const double *__restrict__ rs, double *__restrict__ part, int k) {
+    const BSeg g = bseg(op, tab[m.cs[blockIdx.x]]);
+    const double stopv = g.sc[SC_STOP];
+    const bool half = stopv == STOP_HALF && g.sc[SC_HALFK] == (double)k;
+    if (stopv != 0.0 && !half) return;
+    const double alpha = g.sc[SC_ALPHA], omega = g.sc[SC_OMEGA];
+    const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < g.n) {
+        const long j = g.off + i;
+        const double xh = x[j] + alpha * Mp[j];
+        if (half) {
+            x[j] = xh;
+        } else {
+            const double ri = s[j] - omega * AMs[j];
+            x[j] = xh + omega * Ms[j];
+            r[j] = ri;
+            acc[0] = rs[j] * ri;
+            acc[1] = ri * ri;
+        }
+    }
+    if (half) return;  // uniform per block
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        part[m.nblk + blockIdx.x] = acc[1];
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_b_bicg_p(int op, const BCase *__restrict__ tab, BMapDev m,
+                                                 double *__restrict__ p, const double *__restrict__ r,
+                                                 const double *__restrict__ AMp) {
+    const BSeg g = bseg(op, tab[m.cs[blockIdx.x]]);
+    if (g.sc[SC_STOP] != 0.0) return;
+    const double beta = g.sc[SC_BETA], omega = g.sc[SC_OMEGA];
+    const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
+    if (i < g.n) {
+        const long j = g.off + i;
+        p[j] = r[j] + beta * p[j] - beta * omega * AMp[j];
+    }
+}
+
+// one block per case: the case's block partials in a fixed order, then finish_update
+__global__ void __launch_bounds__(BS) k_b_finish(int op, const BCase *__restrict__ tab, const int *__restrict__ first,
+                                                 int nblk, const double *__restrict__ part, int fin, double rtol,
+                                                 int maxit) {
+    const int c = blockIdx.x;
+    double *sc = bseg(op, tab[c]).sc;
+    if (fin != FIN_INIT && sc[SC_STOP] != 0.0) return;  // uniform per block
+    const int ng = (fin == FIN_ALPHA || fin == FIN_SS) ? 1 : 2;
+    const int b0 = first[c], b1 = first[c + 1];
+    double v[2] = {0, 0};
+    for (int g = 0; g < ng; g++)
+        for (int k = b0 + (int)threadIdx.x; k < b1; k += BS) v[g] += part[g * nblk + k];
+    block_sum<2>(v);
+    if (threadIdx.x != 0) return;
+    const double tot[2] = {v[0], v[1]};
+    if (fin == FIN_INIT) {
+        sc[SC_ITS] = 0.0;
+        sc[SC_HALFK] = -1.0;
+        finish_update(fin, tot, sc, 0.0, maxit);
+        sc[SC_TOL] = rtol * sqrt(sc[SC_BB]);
+        if (sqrt(sc[SC_RR]) <= sc[SC_TOL]) sc[SC_STOP] = STOP_CONVERGED;  // converged at the start
+    } else {
+        finish_update(fin, tot, sc, sc[SC_TOL], maxit);
+    }
+}
+
+void launch_b_bicg_s(int op, const BCase *tab, const BMapDev &m, const double *r, const double *AMp, double *s_,
+                     double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_bicg_s, dim3(m.nblk), dim3(BS), 0, s, op, tab, m, r, AMp, s_, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_bicg_xr(int op, const BCase *tab, const BMapDev &m, double *x, const double *Mp, const double *Ms,
+                      double *r, const double *s_, const double *AMs, const double *rs, double *part, int k,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_b_bicg_xr, dim3(m.nblk), dim3(BS), 0, s, op, tab, m, x, Mp, Ms, r, s_, AMs, rs, part, k);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_bicg_p(int op, const BCase *tab, const BMapDev &m, double *p, const double *r, const double *AMp,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_b_bicg_p, dim3(m.nblk), dim3(BS), 0, s, op, tab, m, p, r, AMp);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_finish(int op, const BCase *tab, int ncases, const int *first, int nblk, const double *part, int fin,
+                     double rtol, int maxit, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_finish, dim3(ncases), dim3(BS), 0, s, op, tab, first, nblk, part, fin, rtol, maxit);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_jacobi(long n, const double *dinv, const double *r, double *z, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi_apply, dim3(grid_for(n)), dim3(BS), 0, s, (int)n, dinv, r, z, nullptr);
     CUIBM_LAUNCH_CHECK();
 }
 

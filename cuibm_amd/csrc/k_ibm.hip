@@ -16,6 +16,7 @@
 // every read sees the field as it was before the update.
 #include <algorithm>
 
+#include "batch.hpp"
 #include "ibm.hpp"
 #include "krylov.hpp"
 
@@ -771,73 +772,85 @@ __device__ __forceinline__ double wsum(double v) {
     return v;
 }
 
+// the force terms of work item t of the box (FadlunModified/kernels/calculateForce.cu:32-283),
+// added into acc[0..5]
+__device__ __forceinline__ void force_term(long t, const double *__restrict__ u, const double *__restrict__ uold,
+                                           const double *__restrict__ p, const int *__restrict__ tagsIn,
+                                           const GridDev &g, double nu, double dt, int I, int J, int ncx, int ncy,
+                                           double (&acc)[6]) {
+    const int nx = g.nx, ny = g.ny;
+    const double *dx = g.dx, *dy = g.dy;
+    const long n0 = ncy, n1 = ncx + 1, n2 = (long)(ncx + 1) * ncy, n3 = ncy + 1, n4 = ncx;
+    long idx = t;
+    if (idx < n0) {  // dragLeftRight
+        const int Ip = (J + idx) * nx + I, Iu = (J + idx) * (nx - 1) + I;
+        acc[0] += -((p[Ip + ncx] - p[Ip]) * dy[J + idx] +
+                    ((u[Iu + ncx] + u[Iu + ncx - 1]) * (u[Iu + ncx] + u[Iu + ncx - 1]) / 4 -
+                     (u[Iu - 1] + u[Iu]) * (u[Iu - 1] + u[Iu]) / 4) * dy[J + idx] -
+                    nu * ((u[Iu + ncx] - u[Iu + ncx - 1]) / dx[I + ncx] - (u[Iu] - u[Iu - 1]) / dx[I]) * dy[J + idx]);
+        return;
+    }
+    idx -= n0;
+    if (idx < n1) {  // dragBottomTop
+        const int Iu = J * (nx - 1) + (I - 1 + idx), Iv = (nx - 1) * ny + (J - 1) * nx + I + idx;
+        acc[1] += -((0.25 * (u[Iu + ncy * (nx - 1)] + u[Iu + (ncy - 1) * (nx - 1)]) * (u[Iv + ncy * nx] + u[Iv + ncy * nx - 1]) -
+                      0.25 * (u[Iu] + u[Iu - (nx - 1)]) * (u[Iv] + u[Iv - 1])) -
+                    nu * (((u[Iu + ncy * (nx - 1)] - u[Iu + (ncy - 1) * (nx - 1)]) / 2.0 / (dy[J + ncy] + dy[J + ncy - 1]) +
+                           (u[Iv + ncy * nx] - u[Iv + ncy * nx - 1]) / 2.0 / (dx[I + idx] + dx[I + idx - 1])) -
+                          ((u[Iu] - u[Iu - (nx - 1)]) / 2.0 / (dy[J] + dy[J - 1]) +
+                           (u[Iv] - u[Iv - 1]) / 2.0 / (dx[I + idx] + dx[I + idx - 1])))) *
+                  0.5 * (dx[I + idx] + dx[I + idx - 1]);
+        return;
+    }
+    idx -= n1;
+    if (idx < n2) {  // dragUnsteady
+        const int i = idx % (ncx + 1), j = idx / (ncx + 1);
+        const int Iu = (J + j) * (nx - 1) + (I - 1 + i);
+        acc[2] += -(tagsIn[Iu] == -1) * ((u[Iu] * dy[J + j] - uold[Iu] * dy[J + j]) / dt * 0.5 * (dx[I + i] + dx[I - 1 + i]));
+        return;
+    }
+    idx -= n2;
+    if (idx < n3) {  // liftLeftRight
+        const int Iu = (J + idx) * (nx - 1) + (I - 1), Iv = (nx - 1) * ny + (J - 1 + idx) * nx + I;
+        acc[3] += -((0.25 * (u[Iu + ncx] + u[Iu + ncx - (nx - 1)]) * (u[Iv + ncx] + u[Iv + ncx - 1]) -
+                      0.25 * (u[Iu] + u[Iu - (nx - 1)]) * (u[Iv] + u[Iv - 1])) -
+                    nu * (((u[Iu + ncx] - u[Iu + ncx - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +
+                           (u[Iv + ncx] - u[Iv + ncx - 1]) / 2.0 / (dx[I + ncx] + dx[I + ncx - 1])) -
+                          ((u[Iu] - u[Iu - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +
+                           (u[Iv] - u[Iv - 1]) / 2.0 / (dx[I] + dx[I - 1])))) *
+                  0.5 * (dy[J + idx] + dy[J - 1 + idx]);
+        return;
+    }
+    idx -= n3;
+    if (idx < n4) {  // liftBottomTop
+        const int Ip = J * nx + I + idx, Iv = (nx - 1) * ny + (J - 1) * nx + I + idx;
+        acc[4] += -((p[Ip + ncy * nx] - p[Ip - nx]) * dx[I + idx] +
+                    (0.25 * (u[Iv + (ncy + 1) * nx] + u[Iv + ncy * nx]) * (u[Iv + (ncy + 1) * nx] + u[Iv + ncy * nx]) -
+                     0.25 * (u[Iv] + u[Iv - nx]) * (u[Iv] + u[Iv - nx])) * dx[I + idx] -
+                    nu * ((u[Iv + (ncy + 1) * nx] - u[Iv + ncy * nx]) * dx[I + idx] / dy[J + ncy] -
+                          (u[Iv] - u[Iv - nx]) * dx[I] / dy[J - 1]));
+        return;
+    }
+    idx -= n4;
+    {  // liftUnsteady
+        const int i = idx % ncx, j = idx / ncx;
+        const int Iv = (J - 1 + j) * nx + (I + i) + (nx - 1) * ny;
+        acc[5] += -(tagsIn[Iv] == -1) * ((u[Iv] * dx[I + i] - uold[Iv] * dx[I + i]) / dt * 0.5 * (dy[J + j] + dy[J - 1 + j]));
+    }
+}
+
+__device__ __forceinline__ long force_items(int ncx, int ncy) {
+    return (long)ncy + (ncx + 1) + (long)(ncx + 1) * ncy + (ncy + 1) + ncx + (long)ncx * (ncy + 1);
+}
+
 __global__ void __launch_bounds__(BS) k_force(const double *__restrict__ u, const double *__restrict__ uold,
                                               const double *__restrict__ p, const int *__restrict__ tagsIn, GridDev g,
                                               double nu, double dt, int I, int J, int ncx, int ncy,
                                               double *__restrict__ part) {
-    const int nx = g.nx, ny = g.ny;
-    const double *dx = g.dx, *dy = g.dy;
-    const long n0 = ncy, n1 = ncx + 1, n2 = (long)(ncx + 1) * ncy, n3 = ncy + 1, n4 = ncx, n5 = (long)ncx * (ncy + 1);
-    const long total = n0 + n1 + n2 + n3 + n4 + n5;
+    const long total = force_items(ncx, ncy);
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-        long idx = t;
-        if (idx < n0) {  // dragLeftRight
-            const int Ip = (J + idx) * nx + I, Iu = (J + idx) * (nx - 1) + I;
-            acc[0] += -((p[Ip + ncx] - p[Ip]) * dy[J + idx] +
-                        ((u[Iu + ncx] + u[Iu + ncx - 1]) * (u[Iu + ncx] + u[Iu + ncx - 1]) / 4 -
-                         (u[Iu - 1] + u[Iu]) * (u[Iu - 1] + u[Iu]) / 4) * dy[J + idx] -
-                        nu * ((u[Iu + ncx] - u[Iu + ncx - 1]) / dx[I + ncx] - (u[Iu] - u[Iu - 1]) / dx[I]) * dy[J + idx]);
-            continue;
-        }
-        idx -= n0;
-        if (idx < n1) {  // dragBottomTop
-            const int Iu = J * (nx - 1) + (I - 1 + idx), Iv = (nx - 1) * ny + (J - 1) * nx + I + idx;
-            acc[1] += -((0.25 * (u[Iu + ncy * (nx - 1)] + u[Iu + (ncy - 1) * (nx - 1)]) * (u[Iv + ncy * nx] + u[Iv + ncy * nx - 1]) -
-                          0.25 * (u[Iu] + u[Iu - (nx - 1)]) * (u[Iv] + u[Iv - 1])) -
-                        nu * (((u[Iu + ncy * (nx - 1)] - u[Iu + (ncy - 1) * (nx - 1)]) / 2.0 / (dy[J + ncy] + dy[J + ncy - 1]) +
-                               (u[Iv + ncy * nx] - u[Iv + ncy * nx - 1]) / 2.0 / (dx[I + idx] + dx[I + idx - 1])) -
-                              ((u[Iu] - u[Iu - (nx - 1)]) / 2.0 / (dy[J] + dy[J - 1]) +
-                               (u[Iv] - u[Iv - 1]) / 2.0 / (dx[I + idx] + dx[I + idx - 1])))) *
-                      0.5 * (dx[I + idx] + dx[I + idx - 1]);
-            continue;
-        }
-        idx -= n1;
-        if (idx < n2) {  // dragUnsteady
-            const int i = idx % (ncx + 1), j = idx / (ncx + 1);
-            const int Iu = (J + j) * (nx - 1) + (I - 1 + i);
-            acc[2] += -(tagsIn[Iu] == -1) * ((u[Iu] * dy[J + j] - uold[Iu] * dy[J + j]) / dt * 0.5 * (dx[I + i] + dx[I - 1 + i]));
-            continue;
-        }
-        idx -= n2;
-        if (idx < n3) {  // liftLeftRight
-            const int Iu = (J + idx) * (nx - 1) + (I - 1), Iv = (nx - 1) * ny + (J - 1 + idx) * nx + I;
-            acc[3] += -((0.25 * (u[Iu + ncx] + u[Iu + ncx - (nx - 1)]) * (u[Iv + ncx] + u[Iv + ncx - 1]) -
-                          0.25 * (u[Iu] + u[Iu - (nx - 1)]) * (u[Iv] + u[Iv - 1])) -
-                        nu * (((u[Iu + ncx] - u[Iu + ncx - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +
-                               (u[Iv + ncx] - u[Iv + ncx - 1]) / 2.0 / (dx[I + ncx] + dx[I + ncx - 1])) -
-                              ((u[Iu] - u[Iu - (nx - 1)]) / 2.0 / (dy[J + idx] + dy[J - 1 + idx]) +
-                               (u[Iv] - u[Iv - 1]) / 2.0 / (dx[I] + dx[I - 1])))) *
-                      0.5 * (dy[J + idx] + dy[J - 1 + idx]);
-            continue;
-        }
-        idx -= n3;
-        if (idx < n4) {  // liftBottomTop
-            const int Ip = J * nx + I + idx, Iv = (nx - 1) * ny + (J - 1) * nx + I + idx;
-            acc[4] += -((p[Ip + ncy * nx] - p[Ip - nx]) * dx[I + idx] +
-                        (0.25 * (u[Iv + (ncy + 1) * nx] + u[Iv + ncy * nx]) * (u[Iv + (ncy + 1) * nx] + u[Iv + ncy * nx]) -
-                         0.25 * (u[Iv] + u[Iv - nx]) * (u[Iv] + u[Iv - nx])) * dx[I + idx] -
-                        nu * ((u[Iv + (ncy + 1) * nx] - u[Iv + ncy * nx]) * dx[I + idx] / dy[J + ncy] -
-                              (u[Iv] - u[Iv - nx]) * dx[I] / dy[J - 1]));
-            continue;
-        }
-        idx -= n4;
-        {  // liftUnsteady
-            const int i = idx % ncx, j = idx / ncx;
-            const int Iv = (J - 1 + j) * nx + (I + i) + (nx - 1) * ny;
-            acc[5] += -(tagsIn[Iv] == -1) * ((u[Iv] * dx[I + i] - uold[Iv] * dx[I + i]) / dt * 0.5 * (dy[J + j] + dy[J - 1 + j]));
-        }
-    }
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x)
+        force_term(t, u, uold, p, tagsIn, g, nu, dt, I, J, ncx, ncy, acc);
     __shared__ double sh[6][BS / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int k = 0; k < 6; k++) {
@@ -987,6 +1000,47 @@ void launch_force(const double *u, const double *uold, const double *p, const in
     hipLaunchKernelGGL(k_force, dim3(FORCE_BLOCKS), dim3(BS), 0, s, u, uold, p, ghostUV, g, nu, dt, I, J, ncx, ncy, part);
     CUIBM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_force_finish, dim3(1), dim3(64), 0, s, part, FORCE_BLOCKS, out);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// ---- lockstep batches: the force of every case, BFORCE_BLOCKS blocks per case ----
+
+__global__ void __launch_bounds__(BS) k_b_force(const BCase *__restrict__ tab, const double *__restrict__ u,
+                                                const double *__restrict__ uold, const double *__restrict__ p,
+                                                double *__restrict__ part) {
+    const int c = blockIdx.x / BFORCE_BLOCKS, lb = blockIdx.x % BFORCE_BLOCKS;
+    const BCase &C = tab[c];
+    const long total = force_items(C.fncx, C.fncy);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (long t = lb * (long)BS + threadIdx.x; t < total; t += (long)BFORCE_BLOCKS * BS)
+        force_term(t, u + C.offUV, uold + C.offUV, p + C.offP, C.ghostUV, C.g, C.nu, C.dt, C.fI, C.fJ, C.fncx, C.fncy,
+                   acc);
+    __shared__ double sh[6][BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < 6; k++) {
+        const double v = wsum(acc[k]);
+        if (lane == 0) sh[k][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        part[(6 * c + k) * BFORCE_BLOCKS + lb] = (sh[k][0] + sh[k][1]) + (sh[k][2] + sh[k][3]);
+    }
+}
+
+__global__ void k_b_force_finish(int ncases, const double *__restrict__ part, double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * ncases) return;
+    double s = 0;
+    for (int b = 0; b < BFORCE_BLOCKS; b++) s += part[t * BFORCE_BLOCKS + b];
+    out[t] = s;
+}
+
+void launch_b_force(const BCase *tab, int ncases, const double *u, const double *uold, const double *p, double *part,
+                    double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_force, dim3(ncases * BFORCE_BLOCKS), dim3(BS), 0, s, tab, u, uold, p, part);
+    CUIBM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_b_force_finish, dim3((6 * ncases + BS - 1) / BS), dim3(BS), 0, s, ncases, part, out);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -8,6 +8,7 @@
 // rhs1 = u + dt(0.5 Nold - 1.5 N + 0.5 L) + bc1 in one pass (L and bc1 are stored
 // only when asked for). The modified-Fadlun row overrides
 // (FadlunModified/kernels/intermediateVelocity.cu:44-63) are fused as an epilogue.
+#include "batch.hpp"
 #include "kernels.hpp"
 
 namespace cuibm {
@@ -127,6 +128,23 @@ __global__ void __launch_bounds__(BS) k_rhs1(const double *__restrict__ u, const
     }
 }
 
+// -div(uhat) at cell ip of the plain solver (NavierStokes/kernels/intermediatePressure.cu:30-71)
+__device__ __forceinline__ double rhs2_plain(const double *__restrict__ uhat, const GridDev &g, const BcDev &b, int ip) {
+    const int nx = g.nx, ny = g.ny;
+    const double *dx = g.dx, *dy = g.dy;
+    const int I = ip % nx, J = ip / nx, iu = (nx - 1) * J + I, iv = (nx - 1) * ny + nx * J + I;
+    double t = 0;
+    if (I != nx - 1) t -= uhat[iu] / dx[I];
+    else t -= b.xp[J] / dx[I];
+    if (I != 0) t += uhat[iu - 1] / dx[I];
+    else t += b.xm[J] / dx[I];
+    if (J != ny - 1) t -= uhat[iv] / dy[J];
+    else t -= b.yp[(nx - 1) + I] / dy[J];
+    if (J != 0) t += uhat[iv - nx] / dy[J];
+    else t += b.ym[(nx - 1) + I] / dy[J];
+    return t;
+}
+
 // rhs2 = -div(uhat) (NavierStokes/kernels/intermediatePressure.cu:30-71); with
 // `fad` set, the modified-Fadlun cut-cell variant (FadlunModified/kernels/intermediatePressure.cu:17-138)
 __global__ void __launch_bounds__(BS) k_rhs2(const double *__restrict__ uhat, double *__restrict__ rhs2, GridDev g,
@@ -164,17 +182,22 @@ __global__ void __launch_bounds__(BS) k_rhs2(const double *__restrict__ uhat, do
             if (G[ip + nx] == 0) t -= uhat[iv] / dy[J];
             if (G[ip - nx] == 0) t += uhat[iv - nx] / dy[J];
         } else {
-            if (I != nx - 1) t -= uhat[iu] / dx[I];
-            else t -= b.xp[J] / dx[I];
-            if (I != 0) t += uhat[iu - 1] / dx[I];
-            else t += b.xm[J] / dx[I];
-            if (J != ny - 1) t -= uhat[iv] / dy[J];
-            else t -= b.yp[(nx - 1) + I] / dy[J];
-            if (J != 0) t += uhat[iv - nx] / dy[J];
-            else t += b.ym[(nx - 1) + I] / dy[J];
+            t = rhs2_plain(uhat, g, b, ip);
         }
         rhs2[ip] = t;
     }
+}
+
+// uhat - dt grad p at velocity node i of the plain solver (NavierStokes/kernels/projectVelocity.cu:25-70)
+__device__ __forceinline__ double project_plain(const double *__restrict__ uhat, const double *__restrict__ p,
+                                                const GridDev &g, double dt, int i) {
+    const int nx = g.nx, ny = g.ny, numU = (nx - 1) * ny;
+    if (i < numU) {
+        const int I = i % (nx - 1), J = i / (nx - 1), ip = J * nx + I;
+        return uhat[i] - dt * (p[ip + 1] - p[ip]) / (0.5 * g.dx[I + 1] + 0.5 * g.dx[I]);
+    }
+    const int k = i - numU, I = k % nx, J = k / nx, ip = J * nx + I;
+    return uhat[i] - dt * (p[ip + nx] - p[ip]) / (0.5 * g.dy[J + 1] + 0.5 * g.dy[J]);
 }
 
 // u = uhat - dt grad p (NavierStokes/kernels/projectVelocity.cu:25-70, writes uold);
@@ -186,18 +209,14 @@ __global__ void __launch_bounds__(BS) k_project(double *__restrict__ u, const do
     const int nx = g.nx, ny = g.ny, numU = (nx - 1) * ny, numUV = numU + nx * (ny - 1);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < numUV; i += gridDim.x * blockDim.x) {
         double nu_;
-        if (i < numU) {
+        if (!ghostP) {
+            nu_ = project_plain(uhat, p, g, dt, i);
+        } else if (i < numU) {
             const int I = i % (nx - 1), J = i / (nx - 1), ip = J * nx + I;
-            if (ghostP)
-                nu_ = uhat[i] - (ghostP[ip + 1] == -1 && ghostP[ip] == -1) * dt * (p[ip + 1] - p[ip]) / (0.5 * g.dx[I + 1] + 0.5 * g.dx[I]);
-            else
-                nu_ = uhat[i] - dt * (p[ip + 1] - p[ip]) / (0.5 * g.dx[I + 1] + 0.5 * g.dx[I]);
+            nu_ = uhat[i] - (ghostP[ip + 1] == -1 && ghostP[ip] == -1) * dt * (p[ip + 1] - p[ip]) / (0.5 * g.dx[I + 1] + 0.5 * g.dx[I]);
         } else {
             const int k = i - numU, I = k % nx, J = k / nx, ip = J * nx + I;
-            if (ghostP)
-                nu_ = uhat[i] - (ghostP[ip + nx] == -1 && ghostP[ip] == -1) * dt * (p[ip + nx] - p[ip]) / (0.5 * g.dy[J + 1] + 0.5 * g.dy[J]);
-            else
-                nu_ = uhat[i] - dt * (p[ip + nx] - p[ip]) / (0.5 * g.dy[J + 1] + 0.5 * g.dy[J]);
+            nu_ = uhat[i] - (ghostP[ip + nx] == -1 && ghostP[ip] == -1) * dt * (p[ip + nx] - p[ip]) / (0.5 * g.dy[J + 1] + 0.5 * g.dy[J]);
         }
         if (!ghostP) uold[i] = u[i];
         u[i] = nu_;
@@ -206,9 +225,9 @@ __global__ void __launch_bounds__(BS) k_project(double *__restrict__ u, const do
 
 // convective outflow at x+ (FadlunModified/kernels/intermediateVelocity.cu:19-41),
 // Uinf = Vinf = 1 as hard-coded in luo_base/intermediateVelocity.cu:13-14
-__global__ void k_robin(const double *__restrict__ u, double *__restrict__ xp, GridDev g, double dt) {
+__device__ __forceinline__ void robin_one(const double *__restrict__ u, double *__restrict__ xp, const GridDev &g,
+                                          double dt, int t) {
     const int nx = g.nx, ny = g.ny, numU = (nx - 1) * ny;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ny) {
         const int J = t, i = J * (nx - 1) + nx - 1;
         const double beta = 1.0 * dt / g.dx[nx - 1];
@@ -218,6 +237,10 @@ __global__ void k_robin(const double *__restrict__ u, double *__restrict__ xp, G
         const double beta = 1.0 * dt / g.dx[nx - 1];
         xp[J + ny] = xp[J + ny] * (1 - beta) + beta * u[i + numU - 1];
     }
+}
+
+__global__ void k_robin(const double *__restrict__ u, double *__restrict__ xp, GridDev g, double dt) {
+    robin_one(u, xp, g, dt, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 static int grid_for(long n) {
@@ -315,6 +338,71 @@ void launch_cfl(const double *u, const GridDev &g, double dt, double *part, doub
 
 void launch_robin(const double *u, double *xp, const GridDev &g, double dt, hipStream_t s) {
     hipLaunchKernelGGL(k_robin, dim3(nblk(2 * g.ny - 1)), dim3(BS), 0, s, u, xp, g, dt);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// ---- lockstep batches: the same sweeps over every case of a batch, a block map giving
+// each block its case and its block within the case ----
+
+__global__ void __launch_bounds__(BS) k_b_robin(const BCase *__restrict__ tab, BMapDev m, const double *__restrict__ u,
+                                                double *__restrict__ xp) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const int t = m.loc[blockIdx.x] * BS + threadIdx.x;
+    if (t < 2 * C.g.ny - 1) robin_one(u + C.offUV, xp + C.offXP, C.g, C.dt, t);
+}
+
+__global__ void __launch_bounds__(BS) k_b_rhs1(const BCase *__restrict__ tab, BMapDev m, const double *__restrict__ u,
+                                               const double *__restrict__ Nold, double *__restrict__ N,
+                                               double *__restrict__ rhs) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
+    if (i >= C.numUV) return;
+    const double *uc = u + C.offUV;
+    const int numU = (C.g.nx - 1) * C.g.ny;
+    double Nv, Lv, Bv;
+    if (i < numU) u_terms(uc, C.g, C.bc, C.nu, C.dt, i, Nv, Lv, Bv);
+    else v_terms(uc, C.g, C.bc, C.nu, C.dt, i - numU, Nv, Lv, Bv);
+    N[C.offUV + i] = Nv;
+    rhs[C.offUV + i] = uc[i] + C.dt * (0.5 * Nold[C.offUV + i] - 1.5 * Nv + 0.5 * Lv) + Bv;
+}
+
+__global__ void __launch_bounds__(BS) k_b_rhs2(const BCase *__restrict__ tab, BMapDev m,
+                                               const double *__restrict__ uhat, double *__restrict__ rhs2) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const int ip = m.loc[blockIdx.x] * BS + threadIdx.x;
+    if (ip < C.numP) rhs2[C.offP + ip] = rhs2_plain(uhat + C.offUV, C.g, C.bc, ip);
+}
+
+__global__ void __launch_bounds__(BS) k_b_project(const BCase *__restrict__ tab, BMapDev m, double *__restrict__ u,
+                                                  const double *__restrict__ uhat, double *__restrict__ uold,
+                                                  const double *__restrict__ p) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
+    if (i >= C.numUV) return;
+    const double v = project_plain(uhat + C.offUV, p + C.offP, C.g, C.dt, i);
+    uold[C.offUV + i] = u[C.offUV + i];
+    u[C.offUV + i] = v;
+}
+
+void launch_b_robin(const BCase *tab, const BMapDev &m, const double *u, double *xp, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_robin, dim3(m.nblk), dim3(BS), 0, s, tab, m, u, xp);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_rhs1(const BCase *tab, const BMapDev &m, const double *u, const double *Nold, double *N, double *rhs,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_b_rhs1, dim3(m.nblk), dim3(BS), 0, s, tab, m, u, Nold, N, rhs);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_rhs2(const BCase *tab, const BMapDev &m, const double *uhat, double *rhs2, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_rhs2, dim3(m.nblk), dim3(BS), 0, s, tab, m, uhat, rhs2);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_project(const BCase *tab, const BMapDev &m, double *u, const double *uhat, double *uold,
+                      const double *p, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_project, dim3(m.nblk), dim3(BS), 0, s, tab, m, u, uhat, uold, p);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 
+#include "batch.hpp"
 #include "stencil.hpp"
 
 namespace cuibm {
@@ -202,13 +203,11 @@ constexpr int N2 = H2 * W2, N1 = H1 * W1, R2 = (N2 + 255) / 256, R1 = (N1 + 255)
 // (b is the FP64 Krylov vector; x2 and r are level vectors of type T); all global loads
 // of a thread are issued before the first is used
 template <class T>
-__global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, T w, const double *__restrict__ b, T *__restrict__ x2g,
-                                                T *__restrict__ rg, int tiles_x, int ntiles) {
+__device__ __forceinline__ void p5_pre_tile(const P5Dev &A, T w, const double *__restrict__ b, T *__restrict__ x2g,
+                                            T *__restrict__ rg, int tiles_x, int t) {
     const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
     __shared__ TileCoef<T> cf;
     __shared__ T sb[N2], swd[N2], sx2[N1];
-    const int t = xcd_band(ntiles);
-    if (t >= ntiles) return;  // uniform per block
     const int I0 = (t % tiles_x) * TX, J0 = (t / tiles_x) * TY;
     load_coef(A, I0, J0, cf);
     const T *dinv = dinv_of<T>(A);
@@ -260,17 +259,23 @@ __global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, T w, const double *__re
     }
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) k_p5_pre(P5Dev A, T w, const double *__restrict__ b, T *__restrict__ x2g,
+                                                T *__restrict__ rg, int tiles_x, int ntiles) {
+    const int t = xcd_band(ntiles);
+    if (t >= ntiles) return;  // uniform per block
+    p5_pre_tile<T>(A, w, b, x2g, rg, tiles_x, t);
+}
+
 // two weighted-Jacobi sweeps on one tile in the compute type T: x (halo 2, a level
 // vector) -> y (halo 1) -> xo (tile, the FP64 Krylov vector); b is the FP64 Krylov vector
 template <class T>
-__global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, T w, const T *__restrict__ x,
-                                                   const double *__restrict__ b, double *__restrict__ xo, int tiles_x,
-                                                   int ntiles) {
+__device__ __forceinline__ void p5_sweep2_tile(const P5Dev &A, T w, const T *__restrict__ x,
+                                               const double *__restrict__ b, double *__restrict__ xo, int tiles_x,
+                                               int t) {
     const double stopv = A.stop ? *A.stop : 0.0;  // the Krylov solve's stop code, tested after the loads
     __shared__ TileCoef<T> cf;
     __shared__ T sx[N2], sb[N1], swd[N1];
-    const int t = xcd_band(ntiles);
-    if (t >= ntiles) return;
     const int I0 = (t % tiles_x) * TX, J0 = (t / tiles_x) * TY;
     load_coef(A, I0, J0, cf);
     const T *dinv = dinv_of<T>(A);
@@ -345,6 +350,15 @@ __global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, T w, const T *__rest
         const T s = t_row(A, cf, li + 2, lj + 2, I, J, d, [&](int o) { return sy[c1 + o]; }, W1);
         xo[ip] = (double)(sy[c1] + swd[c1] * (sb[c1] - s)) + cdefl;
     }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_p5_sweep2(P5Dev A, T w, const T *__restrict__ x,
+                                                   const double *__restrict__ b, double *__restrict__ xo, int tiles_x,
+                                                   int ntiles) {
+    const int t = xcd_band(ntiles);
+    if (t >= ntiles) return;
+    p5_sweep2_tile<T>(A, w, x, b, xo, tiles_x, t);
 }
 
 // partial sums of w_i r_i, w_i = dx[I] dy[J], over a fixed grid of RED_GRID blocks
@@ -626,6 +640,162 @@ void launch_p5_deflate(const P5Dev &A, const double *r, double *part, double *de
 void launch_p5_sweep(const P5Dev &A, double omega, const double *x, const double *b, double *xn, hipStream_t s) {
     const int nblk = (A.nx * A.ny + BS - 1) / BS;
     hipLaunchKernelGGL(k_p5_sweep, dim3(xcd_grid(nblk)), dim3(BS), 0, s, A, omega, x, b, xn, nblk);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// ---- lockstep batches: the stencil operators and the finest AMG level of every case ----
+
+int p5_tiles(int nx, int ny) { return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY); }
+
+template <int OP>
+__device__ __forceinline__ double b_row(const BCase &C, int i, const double *__restrict__ xc) {
+    if (OP == BOP_VEL) return v5_row(C.v5, i, xc);
+    const P5Dev &A = C.p5;
+    const int I = i % A.nx, J = i / A.nx;
+    return p5_row(A, I, J, p5_diag(A, I, J, i), [&](int o) { return xc[i + o]; }, A.nx);
+}
+
+template <int OP>
+__global__ void __launch_bounds__(BS) k_b_init(const BCase *__restrict__ tab, BMapDev m, const double *__restrict__ x,
+                                               const double *__restrict__ b, double *__restrict__ r,
+                                               double *__restrict__ p, double *__restrict__ rs,
+                                               double *__restrict__ part) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const long off = OP == BOP_VEL ? C.offUV : C.offP;
+    const int n = OP == BOP_VEL ? C.numUV : C.numP;
+    const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < n) {
+        const double bi = b[off + i];
+        const double ri = bi - b_row<OP>(C, i, x + off);
+        r[off + i] = ri;
+        p[off + i] = ri;
+        rs[off + i] = ri;
+        acc[0] = ri * ri;
+        acc[1] = bi * bi;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        part[m.nblk + blockIdx.x] = acc[1];
+    }
+}
+
+template <int OP>
+__global__ void __launch_bounds__(BS) k_b_spmv_dot(const BCase *__restrict__ tab, BMapDev m,
+                                                   const double *__restrict__ x, double *__restrict__ y,
+                                                   const double *__restrict__ a, int want_yy,
+                                                   double *__restrict__ part) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const double *sc = OP == BOP_VEL ? C.scV : C.scP;
+    if (sc[SC_STOP] != 0.0) return;  // uniform per block
+    const long off = OP == BOP_VEL ? C.offUV : C.offP;
+    const int n = OP == BOP_VEL ? C.numUV : C.numP;
+    const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < n) {
+        const double yi = b_row<OP>(C, i, x + off);
+        y[off + i] = yi;
+        acc[0] = yi * a[off + i];
+        if (want_yy) acc[1] = yi * yi;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        part[m.nblk + blockIdx.x] = acc[1];
+    }
+}
+
+void launch_b_init(int op, const BCase *tab, const BMapDev &m, const double *x, const double *b, double *r, double *p,
+                   double *rs, double *part, hipStream_t s) {
+    if (op == BOP_VEL)
+        hipLaunchKernelGGL(k_b_init<BOP_VEL>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, b, r, p, rs, part);
+    else
+        hipLaunchKernelGGL(k_b_init<BOP_POIS>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, b, r, p, rs, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_spmv_dot(int op, const BCase *tab, const BMapDev &m, const double *x, double *y, const double *a,
+                       bool want_yy, double *part, hipStream_t s) {
+    if (op == BOP_VEL)
+        hipLaunchKernelGGL(k_b_spmv_dot<BOP_VEL>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, y, a, (int)want_yy, part);
+    else
+        hipLaunchKernelGGL(k_b_spmv_dot<BOP_POIS>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, y, a, (int)want_yy, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// deflation of the pinned constant mode per case (the FP32 V-cycle)
+__global__ void __launch_bounds__(BS) k_b_wdot(const BCase *__restrict__ tab, BMapDev m, const double *__restrict__ r,
+                                               double *__restrict__ part) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    if (C.p5.stop && *C.p5.stop != 0.0) return;  // uniform per block
+    const int ip = m.loc[blockIdx.x] * BS + threadIdx.x;
+    double acc[1] = {0};
+    if (ip < C.numP) acc[0] = (C.g.dx[ip % C.g.nx] * C.g.dy[ip / C.g.nx]) * r[C.offP + ip];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+__global__ void __launch_bounds__(BS) k_b_wdot_finish(const BCase *__restrict__ tab, const int *__restrict__ first,
+                                                      const double *__restrict__ part,
+                                                      const double *__restrict__ inv_wa, double *__restrict__ defl) {
+    const int c = blockIdx.x;
+    const BCase &C = tab[c];
+    if (C.p5.stop && *C.p5.stop != 0.0) return;  // uniform per block
+    double acc[1] = {0};
+    for (int k = first[c] + (int)threadIdx.x; k < first[c + 1]; k += BS) acc[0] += part[k];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) defl[c] = acc[0] * inv_wa[c];
+}
+
+void launch_b_wdot(const BCase *tab, const BMapDev &m, const double *r, double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_wdot, dim3(m.nblk), dim3(BS), 0, s, tab, m, r, part);
+    CUIBM_LAUNCH_CHECK();
+}
+
+void launch_b_wdot_finish(const BCase *tab, int ncases, const int *first, const double *part, const double *inv_wa,
+                          double *defl, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_wdot_finish, dim3(ncases), dim3(BS), 0, s, tab, first, part, inv_wa, defl);
+    CUIBM_LAUNCH_CHECK();
+}
+
+// the finest-level tile kernels, one block per tile of one case
+template <class T>
+__global__ void __launch_bounds__(256) k_b_p5_pre(const BCase *__restrict__ tab, BMapDev m, T w,
+                                                  const double *__restrict__ b, T *__restrict__ x2g,
+                                                  T *__restrict__ rg) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const P5Dev A = C.p5;
+    p5_pre_tile<T>(A, w, b + C.offP, x2g + C.offP, rg + C.offP, (A.nx + TX - 1) / TX, m.loc[blockIdx.x]);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_b_p5_sweep2(const BCase *__restrict__ tab, BMapDev m, T w,
+                                                     const T *__restrict__ x, const double *__restrict__ b,
+                                                     double *__restrict__ xo) {
+    const BCase &C = tab[m.cs[blockIdx.x]];
+    const P5Dev A = C.p5;
+    p5_sweep2_tile<T>(A, w, x + C.offP, b + C.offP, xo + C.offP, (A.nx + TX - 1) / TX, m.loc[blockIdx.x]);
+}
+
+void launch_b_p5_pre(const BCase *tab, const BMapDev &tiles, float omega, const double *b, float *x2, float *r,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_b_p5_pre<float>, dim3(tiles.nblk), dim3(256), 0, s, tab, tiles, omega, b, x2, r);
+    CUIBM_LAUNCH_CHECK();
+}
+void launch_b_p5_pre(const BCase *tab, const BMapDev &tiles, double omega, const double *b, double *x2, double *r,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_b_p5_pre<double>, dim3(tiles.nblk), dim3(256), 0, s, tab, tiles, omega, b, x2, r);
+    CUIBM_LAUNCH_CHECK();
+}
+void launch_b_p5_sweep2(const BCase *tab, const BMapDev &tiles, float omega, const float *x, const double *b,
+                        double *xo, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_p5_sweep2<float>, dim3(tiles.nblk), dim3(256), 0, s, tab, tiles, omega, x, b, xo);
+    CUIBM_LAUNCH_CHECK();
+}
+void launch_b_p5_sweep2(const BCase *tab, const BMapDev &tiles, double omega, const double *x, const double *b,
+                        double *xo, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_p5_sweep2<double>, dim3(tiles.nblk), dim3(256), 0, s, tab, tiles, omega, x, b, xo);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -15,7 +15,7 @@ constexpr int RED_GRID = 2048;  // blocks of every reducing kernel
 // 0 running, STOP_* below otherwise; every kernel of the iteration returns at once when
 // it is set, so the host may queue several iterations before reading the block back.
 enum { SC_BB = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_BETA, SC_SS, SC_RR, SC_STOP, SC_ITS, SC_HALFK, SC_D0, SC_D1,
-       SC_COUNT = 16 };
+       SC_TOL, SC_COUNT = 16 };  // SC_TOL: the absolute tolerance of a batched solve
 enum { STOP_CONVERGED = 1, STOP_HALF = 2, STOP_BREAK_HALF = 3, STOP_BREAK_FULL = 4, STOP_MAXIT = 5 };
 // finish-kernel operations
 enum { FIN_INIT = 0, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_BETA, FIN_DOT2 };
@@ -247,8 +247,14 @@ class AmgPrecond : public Preconditioner {
     void level0_post(double *x2, const double *b, double *x, hipStream_t s);
     double level0_pre_bytes() const;
     double level0_post_bytes() const;
+    // the stencil finest level as the V-cycle sees it (FP32 D^-1, deflation constants),
+    // and 1 / w^T a of the deflation (0 without deflation)
+    const P5Dev &fine_dev() const { return fine_dev_; }
+    double deflation_inv_wa() const;
+    int precision() const { return prm_.precision; }
 
   private:
+    friend class BatchAmg;
     struct Impl;
     std::unique_ptr<Impl> im_;
     AmgParams prm_;

@@ -57,6 +57,7 @@ class NavierStokesSolver {
     const SolveStat &lastSolve1() const { return stat1; }
     const SolveStat &lastSolve2() const { return stat2; }
     int step() const { return timeStep; }
+    long info_cells() const { return (long)nx * ny; }
     // algorithmic HBM bytes of the last step's Poisson solve and whole step
     double poissonBytes() const;
     double velocityBytes() const;
@@ -77,6 +78,7 @@ class NavierStokesSolver {
     const Logger &timers() const { return logger; }
 
   protected:
+    friend class LockstepBatch;
     ParameterDB *db;
     Domain *dom;
     SolverOptions opt;
@@ -161,6 +163,7 @@ class luo_base : public NavierStokesSolver {
     void info(double *out, int n) const override;
 
   protected:
+    friend class LockstepBatch;
     // body state (src/bodies.h)
     std::vector<double> bX, bY, bUB, bVB;  // host copies of the body points / velocities
     std::vector<double> bDX, bDY;          // point offsets from the body centre

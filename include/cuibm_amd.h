@@ -222,6 +222,18 @@ int cuibm_ensemble_run(int n, const char *const *case_folders, int argc, const c
                        int workers, long long *cells, double *cd, double *cl, double *wall_s, double *steps_wall_s,
                        char *err, int err_len);
 
+/* No counterpart in the reference: the same run in lockstep. Every phase of the time step
+ * is one launch over all n cases (fields in concatenated arrays, a block map giving each
+ * block its case); the BiCGSTAB solves keep per-case scalars and stopping tests, so each
+ * case stops at its own tolerance. The cases must be LUO with static bodies at rest,
+ * stencil operators (the default), a DIAGONAL velocity and a SMOOTHED_AGGREGATION Poisson
+ * preconditioner of one AMG precision, and equal tolerances, iteration limits and step
+ * counts; otherwise returns CUIBM_ERR_UNSUPPORTED or CUIBM_ERR_ARGUMENT. `workers` host
+ * threads (1..64) run the cases' set-up. Outputs as cuibm_ensemble_run. */
+int cuibm_ensemble_lockstep(int n, const char *const *case_folders, int argc, const char *const *argv, int steps,
+                            int workers, long long *cells, double *cd, double *cl, double *wall_s,
+                            double *steps_wall_s, char *err, int err_len);
+
 /* ---- linear algebra (replaces cusp::krylov::bicgstab / cusp::precond) ------------- */
 
 /* upload a CSR matrix (n x n, 0-based, sorted columns) */
