@@ -68,7 +68,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5, extra=()):
+def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5, extra=(), median=False):
     """initialise, warm up, time `steps` steps; return a dict of figures"""
     t0 = time.perf_counter()
     s = cu.Solver(case, "-amgPrecision", str(amg_precision), *extra, diagnostics=False)
@@ -103,6 +103,17 @@ def run_form(cu, case, amg_precision, warmup, steps, kernel, reps, probe_steps=5
         out["velocity_bytes_per_step"]
     out["hbm_gbps_step"] = out["step_bytes"] / (out["ms_per_step"] * 1e-3) / 1e9
     s.close()
+    if median:
+        # the same window once more on a fresh instance, each step timed on its own (HIP
+        # events around the step, the stream drained between steps): median and spread
+        s = cu.Solver(case, "-amgPrecision", str(amg_precision), *extra, diagnostics=False)
+        s.initialise()
+        if warmup > 0:
+            s.step_time(warmup)
+        per = sorted(s.time_steps(1) for _ in range(steps))
+        s.close()
+        out["per_step_ms"] = {"median": per[len(per) // 2], "min": per[0], "max": per[-1],
+                              "p90": per[int(0.9 * (len(per) - 1))]}
     if kernel:
         # probes on a second instance of the same form, over the same early steps (W + 1
         # onwards), so that the timed window above carries no events: the roofline kernel
@@ -404,12 +415,13 @@ def main():
         return
     log(f"[bench] host: {host_cpu()}")
     log(f"[bench] mixed form (FP32 V-cycle, FP64 deflation and Krylov): warmup {args.warmup}, steps {args.steps}")
-    mixed = run_form(cu, case, 32, args.warmup, args.steps, args.kernel, args.kernel_reps, args.probe_steps)
+    mixed = run_form(cu, case, 32, args.warmup, args.steps, args.kernel, args.kernel_reps, args.probe_steps,
+                     median=True)
     log(f"[bench] mixed: {mixed['steps_per_s']:.2f} steps/s, Poisson its/step {mixed['poisson_iters_per_step']:.1f}")
     fp64 = None
     if not args.no_fp64:
         log("[bench] FP64 form")
-        fp64 = run_form(cu, case, 64, args.warmup, args.steps, None, 0)
+        fp64 = run_form(cu, case, 64, args.warmup, args.steps, None, 0, median=True)
         log(f"[bench] fp64: {fp64['steps_per_s']:.2f} steps/s, Poisson its/step {fp64['poisson_iters_per_step']:.1f}")
     csr = None
     if not args.no_fp64:
@@ -436,6 +448,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(mixed["ms_per_step"], 4),
+        "per_step_ms": {k: round(v, 4) for k, v in mixed["per_step_ms"].items()},
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
@@ -491,12 +504,14 @@ def main():
         line["fp64"] = {
             "steps_per_s": round(fp64["steps_per_s"], 4),
             "ms_per_step": round(fp64["ms_per_step"], 4),
+            "per_step_ms": {k: round(v, 4) for k, v in fp64["per_step_ms"].items()},
             "poisson_iters_per_step": fp64["poisson_iters_per_step"],
             "hbm_gbps_step": round(fp64["hbm_gbps_step"], 1),
             "cd": fp64["cd"],
             "cl": fp64["cl"],
         }
         line["speedup_mixed_vs_fp64"] = round(mixed["steps_per_s"] / fp64["steps_per_s"], 4)
+        line["speedup_mixed_vs_fp64_median"] = round(fp64["per_step_ms"]["median"] / mixed["per_step_ms"]["median"], 4)
         line["speedup_target"] = 1.7
         line["abs_dcd_mixed_vs_fp64"] = abs(mixed["cd"] - fp64["cd"])
         # where the time goes: the Poisson solve's share of each form's step, and its bytes

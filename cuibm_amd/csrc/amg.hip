@@ -56,11 +56,13 @@ template <class V, class X, int LPR, int OP>
 __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                             const V *__restrict__ v, const X *__restrict__ x, const X *__restrict__ b,
                                             const V *__restrict__ dinv, X *__restrict__ y, X omega,
-                                            const double *__restrict__ stop) {
+                                            const double *__restrict__ stop, const int *__restrict__ seg,
+                                            const double *const *__restrict__ cstop) {
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+        if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
         for (int k = rp[i] + lane; k < e; k += LPR) s += (X)v[k] * x[ci[k]];
@@ -83,11 +85,13 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
 template <class V, class X, int LPR>
 __global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restrict__ rp, const int *__restrict__ ci,
                                              const V *__restrict__ v, const X *__restrict__ x, const X *__restrict__ b,
-                                             X *__restrict__ y, const double *__restrict__ stop) {
+                                             X *__restrict__ y, const double *__restrict__ stop,
+                                             const int *__restrict__ seg, const double *const *__restrict__ cstop) {
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+        if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
         for (int k = rp[i] + lane; k < e; k += LPR) {
@@ -103,17 +107,18 @@ __global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restric
 }
 
 template <class V, class X>
-void launch_rres(int lpr, const DCsr<V> &G, int nf, const X *x, const X *b, X *y, hipStream_t s, const double *stop) {
+void launch_rres(int lpr, const DCsr<V> &G, int nf, const X *x, const X *b, X *y, hipStream_t s, const double *stop,
+                 const int *seg = nullptr, const double *const *cstop = nullptr) {
     const int n = G.n;
     if (n <= 0) return;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_rres<V, X, 1>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        case 4: hipLaunchKernelGGL((k_rres<V, X, 4>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        case 8: hipLaunchKernelGGL((k_rres<V, X, 8>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        case 16: hipLaunchKernelGGL((k_rres<V, X, 16>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
-        default: hipLaunchKernelGGL((k_rres<V, X, 32>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop); break;
+        case 1: hipLaunchKernelGGL((k_rres<V, X, 1>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop, seg, cstop); break;
+        case 4: hipLaunchKernelGGL((k_rres<V, X, 4>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop, seg, cstop); break;
+        case 8: hipLaunchKernelGGL((k_rres<V, X, 8>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop, seg, cstop); break;
+        case 16: hipLaunchKernelGGL((k_rres<V, X, 16>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop, seg, cstop); break;
+        default: hipLaunchKernelGGL((k_rres<V, X, 32>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop, seg, cstop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -126,11 +131,13 @@ template <class V, class X, int LPR>
 __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                              const V *__restrict__ v, const V *__restrict__ dinv,
                                              const X *__restrict__ b, X *__restrict__ x2, X omega,
-                                             const double *__restrict__ stop) {
+                                             const double *__restrict__ stop, const int *__restrict__ seg,
+                                             const double *const *__restrict__ cstop) {
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
     for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+        if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
         for (int k = rp[i] + lane; k < e; k += LPR) {
@@ -150,16 +157,16 @@ __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, 
 
 template <class V, class X>
 void launch_pre2(int lpr, const DCsr<V> &A, const V *dinv, const X *b, X *x2, X omega, hipStream_t s,
-                 const double *stop) {
+                 const double *stop, const int *seg = nullptr, const double *const *cstop = nullptr) {
     const int n = A.n;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_pre2<V, X, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        case 4: hipLaunchKernelGGL((k_pre2<V, X, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        case 8: hipLaunchKernelGGL((k_pre2<V, X, 8>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        case 16: hipLaunchKernelGGL((k_pre2<V, X, 16>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
-        default: hipLaunchKernelGGL((k_pre2<V, X, 32>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop); break;
+        case 1: hipLaunchKernelGGL((k_pre2<V, X, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop, seg, cstop); break;
+        case 4: hipLaunchKernelGGL((k_pre2<V, X, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop, seg, cstop); break;
+        case 8: hipLaunchKernelGGL((k_pre2<V, X, 8>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop, seg, cstop); break;
+        case 16: hipLaunchKernelGGL((k_pre2<V, X, 16>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop, seg, cstop); break;
+        default: hipLaunchKernelGGL((k_pre2<V, X, 32>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop, seg, cstop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -176,17 +183,17 @@ int lanes_for(const HCsr &A) {
 
 template <class V, class X, int OP>
 void launch_csr_op(int lpr, const DCsr<V> &A, const X *x, const X *b, const V *dinv, X *y, X omega, hipStream_t s,
-                   const double *stop = nullptr) {
+                   const double *stop = nullptr, const int *seg = nullptr, const double *const *cstop = nullptr) {
     const int n = A.n;
     if (n <= 0) return;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
     const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
     switch (lpr) {
-        case 1: hipLaunchKernelGGL((k_csr<V, X, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        case 4: hipLaunchKernelGGL((k_csr<V, X, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        case 8: hipLaunchKernelGGL((k_csr<V, X, 8, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        case 16: hipLaunchKernelGGL((k_csr<V, X, 16, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
-        default: hipLaunchKernelGGL((k_csr<V, X, 32, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop); break;
+        case 1: hipLaunchKernelGGL((k_csr<V, X, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop, seg, cstop); break;
+        case 4: hipLaunchKernelGGL((k_csr<V, X, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop, seg, cstop); break;
+        case 8: hipLaunchKernelGGL((k_csr<V, X, 8, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop, seg, cstop); break;
+        case 16: hipLaunchKernelGGL((k_csr<V, X, 16, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop, seg, cstop); break;
+        default: hipLaunchKernelGGL((k_csr<V, X, 32, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop, seg, cstop); break;
     }
     CUIBM_LAUNCH_CHECK();
 }
@@ -253,13 +260,15 @@ struct DenseCase {
     const V *inv;
     int n, ld;
     long off;
+    const double *stop;  // the case's Krylov stop code: the solve is skipped once it is set
 };
 
 template <class V, class X>
 __global__ void __launch_bounds__(256) k_b_dense(const DenseCase<V> *__restrict__ tab, BMapDev m,
                                                  const X *__restrict__ b, X *__restrict__ x) {
     const DenseCase<V> &D = tab[m.cs[blockIdx.x]];
-    dense_row<V, X>(D.n, D.ld, D.inv, b + D.off, x + D.off, m.loc[blockIdx.x] * 4 + (threadIdx.x >> 6), 0.0);
+    dense_row<V, X>(D.n, D.ld, D.inv, b + D.off, x + D.off, m.loc[blockIdx.x] * 4 + (threadIdx.x >> 6),
+                    D.stop ? *D.stop : 0.0);
 }
 
 }  // namespace
@@ -772,6 +781,7 @@ struct BatchDev {
         DArray<X> b, x, xt, r;
         DArray<DenseCase<V>> dense;
         BMap dmap;
+        DArray<int> seg;  // the case of each BATCH_PAD-row segment of the level vectors
     };
     std::vector<Lvl> lv;  // lv[0]: the finest level (P0, R0 and its two vectors)
     int ncases = 0;
@@ -783,8 +793,10 @@ struct BatchDev {
     double *defl = nullptr;
     bool deflate = false;
     double omega = 0.8;
+    DArray<const double *> cstop;  // each case's Poisson stop code: the CSR levels skip stopped cases
 
-    void build(const std::vector<AmgDev<V, X> *> &cd, const std::vector<long> &offP, hipStream_t s) {
+    void build(const std::vector<AmgDev<V, X> *> &cd, const std::vector<long> &offP,
+               const std::vector<const double *> &stops, hipStream_t s) {
         ncases = (int)cd.size();
         int depth = 0;
         for (auto *d : cd) depth = std::max(depth, (int)d->lv.size());
@@ -806,6 +818,14 @@ struct BatchDev {
                 }
             lv[l].n = (int)o;
         }
+        cstop.upload(std::vector<const double *>(stops.begin(), stops.end()), s);
+        for (int l = 0; l < depth; l++) {
+            std::vector<int> seg;
+            for (int c = 0; c < ncases; c++)
+                if ((int)cd[c]->lv.size() > l) seg.insert(seg.end(), batch_pad(cd[c]->lv[l].n) / BATCH_PAD, c);
+            if ((long)seg.size() * BATCH_PAD != lv[l].n) throw Error("batched V-cycle: level segments", ERR_GENERIC);
+            lv[l].seg.upload(seg.empty() ? std::vector<int>{0} : seg, s);
+        }
         for (int l = 0; l < depth; l++) {
             Lvl &L = lv[l];
             HCat<V> A, G, P, R;
@@ -819,7 +839,7 @@ struct BatchDev {
                 if ((int)d.lv.size() <= l) continue;
                 const int rows = (int)batch_pad(d.lv[l].n);
                 if ((int)d.lv.size() - 1 == l) {  // this case's coarsest level
-                    dense.push_back(DenseCase<V>{d.coarse_inv.p(), d.nc, d.ld, off[l][c]});
+                    dense.push_back(DenseCase<V>{d.coarse_inv.p(), d.nc, d.ld, off[l][c], stops[c]});
                     dcount.push_back(d.nc);
                     continue;
                 }
@@ -887,12 +907,16 @@ struct BatchDev {
         const X w = (X)omega;
         if (L.nS > 0) {
             Lvl &C = lv[l + 1];
-            launch_pre2<V, X>(L.lanesA, L.A, L.dinv.p(), L.b.p(), L.xt.p(), w, s, nullptr);
-            launch_rres<V, X>(L.lanesG, L.G, L.nS, L.xt.p(), L.b.p(), C.b.p(), s, nullptr);
+            const double *const *cs = cstop.p();
+            launch_pre2<V, X>(L.lanesA, L.A, L.dinv.p(), L.b.p(), L.xt.p(), w, s, nullptr, L.seg.p(), cs);
+            launch_rres<V, X>(L.lanesG, L.G, L.nS, L.xt.p(), L.b.p(), C.b.p(), s, nullptr, C.seg.p(), cs);
             vcycle(l + 1, s);
-            launch_csr_op<V, X, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), X(0), s);
-            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), L.b.p(), L.dinv.p(), L.r.p(), w, s);
-            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.r.p(), L.b.p(), L.dinv.p(), L.x.p(), w, s);
+            launch_csr_op<V, X, OP_PROLONG>(L.lanesP, L.P, C.x.p(), nullptr, nullptr, L.xt.p(), X(0), s, nullptr,
+                                            L.seg.p(), cs);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.xt.p(), L.b.p(), L.dinv.p(), L.r.p(), w, s, nullptr,
+                                           L.seg.p(), cs);
+            launch_csr_op<V, X, OP_JACOBI>(L.lanesA, L.A, L.r.p(), L.b.p(), L.dinv.p(), L.x.p(), w, s, nullptr,
+                                           L.seg.p(), cs);
         }
         if (L.dmap.nblk() > 0) {
             hipLaunchKernelGGL((k_b_dense<V, X>), dim3(L.dmap.nblk()), dim3(256), 0, s, L.dense.p(), L.dmap.dev(),
@@ -909,9 +933,11 @@ struct BatchDev {
             launch_b_wdot_finish(tab, ncases, pfirst, part.p(), inv_wa.p(), defl, s);
         }
         launch_b_p5_pre(tab, tiles, w, b, F.xt.p(), F.r.p(), s);
-        launch_csr_op<V, X, OP_SPMV>(F.lanesR, F.R, F.r.p(), nullptr, nullptr, C.b.p(), X(0), s);
+        launch_csr_op<V, X, OP_SPMV>(F.lanesR, F.R, F.r.p(), nullptr, nullptr, C.b.p(), X(0), s, nullptr, C.seg.p(),
+                                     cstop.p());
         vcycle(1, s);
-        launch_csr_op<V, X, OP_PROLONG>(F.lanesP, F.P, C.x.p(), nullptr, nullptr, F.xt.p(), X(0), s);
+        launch_csr_op<V, X, OP_PROLONG>(F.lanesP, F.P, C.x.p(), nullptr, nullptr, F.xt.p(), X(0), s, nullptr,
+                                        F.seg.p(), cstop.p());
         launch_b_p5_sweep2(tab, tiles, w, F.xt.p(), b, x, s);
     }
 };
@@ -943,12 +969,14 @@ BatchAmg::BatchAmg(const std::vector<AmgPrecond *> &amg, const std::vector<long>
             throw Error("batched V-cycle: deflation on some cases only", ERR_ARGUMENT);
         iw[c] = amg[c]->deflation_inv_wa();
     }
+    std::vector<const double *> stops(n);
+    for (int c = 0; c < n; c++) stops[c] = tab_host[c].p5.stop;
     auto setup = [&](auto &B, const auto &cd) {
         B.omega = amg[0]->prm_.omega;
         B.tab = tab_dev;
         B.pmap = pmap;
         B.pfirst = pfirst;
-        B.build(cd, offP, s);
+        B.build(cd, offP, stops, s);
         B.tilemap.build(tcount, 1, s);
         B.tiles = B.tilemap.dev();
         B.deflate = deflate;

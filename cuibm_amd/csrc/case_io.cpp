@@ -301,6 +301,12 @@ void readInputs(const std::string &caseFolder, int argc, const char *const *argv
             DB.PoissonSolve.amgPrecision = (prec == 32 || prec == 3264) ? prec : 64;
             i++;
         }
+        else if (a == "-amgCoarseMax") {
+            const int m = std::atoi(v);
+            if (m < 16 || m > 6144) throw YamlError("-amgCoarseMax takes 16..6144");
+            DB.PoissonSolve.amgCoarseMax = m;
+            i++;
+        }
         else if (a == "-poissonPreconditioner") { DB.PoissonSolve.preconditioner = preconditionerFromString(v); i++; }
         else if (a == "-operators") {
             const std::string o = v;

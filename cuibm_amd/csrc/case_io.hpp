@@ -34,6 +34,8 @@ struct LinearSolverParams {
     // 64: FP64 smoothed-aggregation hierarchy; 32: V-cycle in FP32 (matrix values, level
     // vectors, smoother arithmetic); 3264: FP32 matrix values with FP64 vectors
     int amgPrecision = 64;
+    // largest coarsest AMG level (a dense inverse); 0: the solver's default
+    int amgCoarseMax = 0;
 };
 
 // typed restatement of the parameterDB filled by src/io/io.cu:145-197 and the

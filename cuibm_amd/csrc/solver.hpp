@@ -120,7 +120,7 @@ class NavierStokesSolver {
     virtual bool mixedScaleLHS2() const { return false; }
     // largest coarsest AMG level of the Poisson hierarchy (solved with a dense inverse,
     // whose set-up costs 2 n^3)
-    virtual int amgCoarseMax() const { return 2500; }
+    virtual int amgCoarseMax() const { return db->PoissonSolve.amgCoarseMax > 0 ? db->PoissonSolve.amgCoarseMax : 2500; }
     // the Poisson hierarchy is rebuilt every step (its set-up then skips the fused
     // restriction's extra product)
     virtual bool amgRebuiltEachStep() const { return false; }
@@ -263,7 +263,9 @@ class luoIter : public luo_base {
     bool mixedScaleLHS2() const override { return true; }
     // a moving body rebuilds the hierarchy every step: a small coarsest level keeps the
     // dense inverse cheap (2 * 300^3 flops against up to 2 * 2500^3)
-    int amgCoarseMax() const override { return moving ? 300 : 2500; }
+    int amgCoarseMax() const override {
+        return db->PoissonSolve.amgCoarseMax > 0 ? db->PoissonSolve.amgCoarseMax : moving ? 300 : 2500;
+    }
     bool amgRebuiltEachStep() const override { return moving; }
     // row maps (-1 plain, t hybrid slot, -(t + 2) ghost slot), row buffers, row scales
     DArray<int> map1, map2, cnt, rowc, stats, scanPart;

@@ -48,7 +48,8 @@ const char *cuibm_last_error(void);
  * overrides in argv (-nu -uPerturb -vPerturb -scaleCV -nsave -nt -dt -velocityTol
  * -poissonTol, plus -amgPrecision 64|32|3264 (as cuibm_bicgstab's amg_precision; in a
  * solver whose Poisson operator has its stencil form, 32 also removes the pinned constant
- * mode in FP64 around the FP32 V-cycle), -poissonPreconditioner NAME,
+ * mode in FP64 around the FP32 V-cycle), -amgCoarseMax N (largest coarsest AMG level, solved
+ * with a dense inverse: 16..6144, default 2500), -poissonPreconditioner NAME,
  * -operators csr|stencil, -outputFolder DIR and -output 0|1) and construct the solver
  * (not yet initialised). */
 int cuibm_create(const char *case_folder, int argc, const char *const *argv, cuibm_solver **out);

@@ -115,6 +115,15 @@ def test_operators_switch_is_parsed():
     assert e.value.code == -2 and "csr or stencil" in str(e.value)
 
 
+def test_amg_coarse_max_is_parsed_and_bounded():
+    s = cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-amgCoarseMax", "400")
+    s.close()
+    for bad in ("8", "10000"):
+        with pytest.raises(cuibm_amd.CuibmError) as e:
+            cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-amgCoarseMax", bad)
+        assert e.value.code == -2 and "16..6144" in str(e.value)
+
+
 def test_csr_create_rejects_a_malformed_row_pointer():
     """row pointers that do not start at 0 or that decrease are refused before any upload"""
     import ctypes as C
