@@ -44,7 +44,9 @@ INFO_KEYS = ["timeStep", "iter1", "iter2", "res1", "res2", "forceX", "forceY", "
              # velocity of the body centre (VIV)
              "sc_passes", "centerV",
              # largest CFL number so far (NavierStokes/CFL.cu:10-35), its cell and time step
-             "cfl_max", "cfl_I", "cfl_J", "cfl_ts"]
+             "cfl_max", "cfl_I", "cfl_J", "cfl_ts",
+             # moving embedded body: hierarchies set up again on the device (same pattern)
+             "amg_refreshes"]
 
 
 class CuibmError(RuntimeError):
@@ -238,8 +240,8 @@ class Solver:
         _check(lib().cuibm_set_field(self._h, name.encode(), a.ctypes.data_as(C.c_void_p), a.size))
 
     def info(self):
-        out = np.zeros(46)
-        _check(lib().cuibm_info(self._h, _dp(out), 46))
+        out = np.zeros(47)
+        _check(lib().cuibm_info(self._h, _dp(out), 47))
         return dict(zip(INFO_KEYS, out[: len(INFO_KEYS)]))
 
     def timers(self):

@@ -477,12 +477,33 @@ struct AmgDev {
     }
 };
 
+// the device copy of the hierarchy's symbolic structure and FP64 values, for numeric
+// re-set-up (AmgPrecond::refresh)
+struct RefreshLevel {
+    int n = 0;
+    DArray<int> Arp, Aci, Prp, Pci, Rrp, Rci, agg, perm;
+    DArray<int> Prow;  // the row of each slot of P
+    // the products A P and R (A P) (the next level's A) as plans: slot q of the product
+    // sums X[pl[t].x] * Y[pl[t].y] over t = off[q] .. off[q + 1] - 1, in the host order
+    DArray<int> APoff, Coff;
+    DArray<int2> APpl, Cpl;
+    int nnzAP = 0, nnzC = 0;
+    DArray<double> Av, Pv, Rv, APv, tval, d;
+};
+struct RefreshData {
+    bool ready = false;
+    std::vector<RefreshLevel> lv;  // every level but the coarsest, then the coarsest (A only)
+    DArray<double> rowsum, part, w;  // prolongator smoothing: row sums, partial maxima, omega_p
+    DArray<int> flag;
+};
+
 struct AmgPrecond::Impl {
     AmgDev<double, double> d64;
     AmgDev<float, double> d3264;
     AmgDev<float, float> d32;
     std::vector<HostAmgLevel> host;
     KernelProbe probe;
+    RefreshData rf;
     template <class F>
     void each(F f) {
         f(d64);
@@ -493,11 +514,19 @@ struct AmgPrecond::Impl {
 
 KernelProbe &AmgPrecond::pre_probe() { return im_->probe; }
 
+static void refresh_setup(RefreshData &rf, const std::vector<HostAmgLevel> &host, hipStream_t s);
+
 AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine)
     : im_(new Impl), prm_(p) {
     if (p.precision != 64 && p.precision != 32 && p.precision != 3264)
         throw Error("AMG precision must be 64, 32 or 3264", ERR_ARGUMENT);
-    im_->host = amg_build(A, p.coarse_max, p.max_levels, p.scale0, p.fused_restriction);
+    // a hierarchy set up again on the device for moving bodies is built on a padded
+    // pattern that the moved body's rows keep to (the padding entries are zeros, so the
+    // values, aggregates and levels are those of A's own hierarchy)
+    const bool pad = p.refreshable && fine == nullptr && !p.fused_restriction && p.pad_nx > 0;
+    HCsr padded;
+    if (pad) padded = pad_pattern(A, p.pad_nx, p.pad_margin);
+    im_->host = amg_build(pad ? padded : A, p.coarse_max, p.max_levels, p.scale0, p.fused_restriction);
     fine_ = fine != nullptr && im_->host.size() > 1 && p.npre == 2 && p.npost == 2;
     if (fine_) {
         if (fine->nx * fine->ny != A.n) throw Error("stencil and matrix sizes differ", ERR_ARGUMENT);
@@ -511,6 +540,7 @@ AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P
         d.build(im_->host, s);
         if (fine_) fine_dev_.dinv32 = d.fine.dinv32;
     };
+    if (p.refreshable && !fine_ && !p.fused_restriction && im_->host.size() >= 2) refresh_setup(im_->rf, im_->host, s);
     if (p.precision == 32 && fine_) {
         // The FP32 V-cycle loses the weakly pinned constant mode of the Poisson operator
         // (A = A0 + E with A0 1 = 0 and E nonzero only at the pinned cells), and BiCGSTAB
@@ -714,6 +744,386 @@ double AmgPrecond::bytes_per_apply() const {
     double bytes = 0;
     for (double b : launch_bytes()) bytes += b;
     return bytes;
+}
+
+// ---- numeric re-set-up of the hierarchy on the device (same pattern, new values) ----
+
+namespace {
+
+// A's entries into the kept finest pattern S (zeros elsewhere), one row per thread;
+// flag = 1 where an entry of A has no slot in S (columns of S rows are sorted)
+__global__ void k_rf_scatter(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                             const double *__restrict__ v, const int *__restrict__ Srp, const int *__restrict__ Sci,
+                             double *__restrict__ Sv, int *__restrict__ flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int s0 = Srp[i], s1 = Srp[i + 1];
+        for (int q = s0; q < s1; q++) Sv[q] = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            const int c = ci[k];
+            int lo = s0, hi = s1 - 1, at = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int m = Sci[mid];
+                if (m == c) {
+                    at = mid;
+                    break;
+                }
+                if (m < c) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            if (at < 0) flag[0] = 1;
+            else Sv[at] += v[k];
+        }
+    }
+}
+
+// Inverse of the coarsest operator (n <= RF_GJ_MAX) by Gauss-Jordan elimination with
+// partial pivoting in one workgroup, M and X = I in LDS: every value goes through
+// dense_inverse's operations in its order (the pivot is the first row of largest
+// magnitude; entries left of and at the pivot column are not read again and are not
+// updated). flag = 1 on a zero pivot.
+constexpr int RF_GJ_MAX = 60;
+template <class V>
+__global__ void __launch_bounds__(1024) k_rf_gj(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                const double *__restrict__ v, V *__restrict__ inv, int ld,
+                                                int *__restrict__ flag) {
+    __shared__ double M[RF_GJ_MAX * RF_GJ_MAX], X[RF_GJ_MAX * RF_GJ_MAX];
+    __shared__ int piv_s;
+    const int t = threadIdx.x, nt = blockDim.x, nn = n * n;
+    for (int e = t; e < nn; e += nt) {
+        M[e] = 0.0;
+        X[e] = (e / n == e % n) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (t < n)
+        for (int k = rp[t]; k < rp[t + 1]; k++) M[t * n + ci[k]] += v[k];
+    __syncthreads();
+    for (int c = 0; c < n; c++) {
+        if (t < 64) {  // wave 0: lanes hold rows c .. c + 63 (n - c <= 60)
+            const int r = c + t;
+            double a = r < n ? fabs(M[r * n + c]) : -1.0;
+            int idx = r < n ? r : n;
+            for (int o = 32; o > 0; o >>= 1) {
+                const double a2 = __shfl_xor(a, o);
+                const int i2 = __shfl_xor(idx, o);
+                if (a2 > a || (a2 == a && i2 < idx)) {
+                    a = a2;
+                    idx = i2;
+                }
+            }
+            if (t == 0) {
+                piv_s = idx;
+                if (M[idx * n + c] == 0.0) flag[0] = 1;
+            }
+        }
+        __syncthreads();
+        const int piv = piv_s;
+        if (piv != c && t < 2 * n) {
+            double *A = t < n ? M : X;
+            const int k = t < n ? t : t - n;
+            const double x = A[c * n + k];
+            A[c * n + k] = A[piv * n + k];
+            A[piv * n + k] = x;
+        }
+        __syncthreads();
+        const double iv = 1.0 / M[c * n + c];
+        if (t < 2 * n) {
+            const int k = t < n ? t : t - n;
+            if (t >= n) X[c * n + k] = X[c * n + k] * iv;
+            else if (k > c) M[c * n + k] = M[c * n + k] * iv;
+        }
+        __syncthreads();
+        for (int e = t; e < 2 * nn; e += nt) {
+            const bool isx = e >= nn;
+            const int ee = isx ? e - nn : e;
+            const int r = ee / n, k = ee % n;
+            if (r == c || (!isx && k <= c)) continue;
+            const double f = M[r * n + c];
+            if (f == 0.0) continue;
+            if (isx) X[r * n + k] = X[r * n + k] - f * X[c * n + k];
+            else M[r * n + k] = M[r * n + k] - f * M[c * n + k];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < nn; e += nt) inv[(size_t)(e / n) * ld + e % n] = (V)X[e];
+}
+
+// d_i = sum of the diagonal entries of row i; s_i = sum_k |a_ik (1 / d_i)| (amg_build's order)
+__global__ void k_rf_diag(int n, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+                          double *__restrict__ d, double *__restrict__ rs) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double di = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++)
+            if (ci[k] == i) di += v[k];
+        double t = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) t += fabs(v[k] * (1.0 / di));
+        d[i] = di;
+        rs[i] = t;
+    }
+}
+
+// w = 4 / (3 max_i s_i): partial maxima over a fixed grid, then one block
+__global__ void __launch_bounds__(BS) k_rf_max(int n, const double *__restrict__ rs, double *__restrict__ part) {
+    __shared__ double sh[BS];
+    double m = 0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) m = fmax(m, rs[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void __launch_bounds__(BS) k_rf_omega(int nb, const double *__restrict__ part, double *__restrict__ w) {
+    __shared__ double sh[BS];
+    double m = 0;
+    for (int b = threadIdx.x; b < nb; b += BS) m = fmax(m, part[b]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) w[0] = 4.0 / (3.0 * sh[0]);
+}
+
+// P = (I - w D^-1 A) T on P's cached pattern, one thread per slot: for slot (i, c),
+// (A T)_ic summed in the order of row i of A, then -w (1 / d_i) (A T)_ic, plus t_i where c
+// is row i's own aggregate
+__global__ void k_rf_p(int nnz, const int *__restrict__ Prow, const int *__restrict__ Pci,
+                       const int *__restrict__ Arp, const int *__restrict__ Aci, const double *__restrict__ Av,
+                       const int *__restrict__ agg, const double *__restrict__ tval, const double *__restrict__ d,
+                       const double *__restrict__ w, double *__restrict__ Pv) {
+    const double wp = w[0];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
+        const int i = Prow[q], c = Pci[q];
+        double at = 0;
+        bool any = false;
+        for (int k = Arp[i]; k < Arp[i + 1]; k++) {
+            const int j = Aci[k];
+            if (agg[j] == c) {
+                at += Av[k] * tval[j];
+                any = true;
+            }
+        }
+        if (!any) {  // only the tentative entry
+            Pv[q] = tval[i];
+            continue;
+        }
+        const double val = -wp * (1.0 / d[i]) * at;
+        Pv[q] = c == agg[i] ? tval[i] + val : val;
+    }
+}
+
+// R = P^T through the cached permutation, scaled by 1 / d_col on a row-scaled finest level
+__global__ void k_rf_r(int nnz, const int *__restrict__ perm, const double *__restrict__ Pv,
+                       const int *__restrict__ Rci, const double *__restrict__ d, double *__restrict__ Rv) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
+        const double v = Pv[perm[q]];
+        Rv[q] = d ? v * (1.0 / d[Rci[q]]) : v;
+    }
+}
+
+// Z = X Y on Z's cached pattern through its plan, one thread per slot: the products of
+// the slot's (X, Y) entry pairs are summed in the order the host product accumulates them
+__global__ void k_rf_plan(int nnz, const int *__restrict__ off, const int2 *__restrict__ pl,
+                          const double *__restrict__ Xv, const double *__restrict__ Yv, double *__restrict__ Zv) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
+        double acc = 0;
+        for (int t = off[q]; t < off[q + 1]; t++) {
+            const int2 e = pl[t];
+            acc += Xv[e.x] * Yv[e.y];
+        }
+        Zv[q] = acc;
+    }
+}
+
+template <class V>
+__global__ void k_rf_cast(long n, const double *__restrict__ x, V *__restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = (V)x[i];
+}
+template <class V>
+__global__ void k_rf_dinv(int n, const double *__restrict__ d, V *__restrict__ dinv) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dinv[i] = (V)(1.0 / d[i]);
+}
+
+int rf_blocks(long n) { return (int)std::max(1L, std::min((n + BS - 1) / BS, 4096L)); }
+
+template <class T>
+void upload_v(DArray<T> &d, const std::vector<T> &h, hipStream_t s) {
+    d.upload(h.empty() ? std::vector<T>{T(0)} : h, s);
+}
+
+std::vector<int> slot_rows(const std::vector<int> &rp) {
+    std::vector<int> r(rp.empty() ? 0 : rp.back());
+    for (size_t i = 0; i + 1 < rp.size(); i++)
+        for (int q = rp[i]; q < rp[i + 1]; q++) r[q] = (int)i;
+    return r;
+}
+
+// the plan of Z = X Y on Z's pattern (multiply()'s): for each slot of Z, the pairs (entry
+// of X row i, entry of Y row k) contributing to it, in X row order as multiply() sums them
+void product_plan(const HCsr &X, const HCsr &Y, const std::vector<int> &Zrp, const std::vector<int> &Zci,
+                  std::vector<int> &off, std::vector<int2> &pl) {
+    const size_t nnz = Zci.size();
+    std::vector<int> pos(std::max(Y.ncols, 1), -1), cnt(nnz + 1, 0);
+    auto each = [&](auto f) {
+        for (int i = 0; i < X.n; i++) {
+            for (int q = Zrp[i]; q < Zrp[i + 1]; q++) pos[Zci[q]] = q;
+            for (int ka = X.rp[i]; ka < X.rp[i + 1]; ka++) {
+                const int k = X.ci[ka];
+                for (int kb = Y.rp[k]; kb < Y.rp[k + 1]; kb++) f(pos[Y.ci[kb]], ka, kb);
+            }
+            for (int q = Zrp[i]; q < Zrp[i + 1]; q++) pos[Zci[q]] = -1;
+        }
+    };
+    each([&](int q, int, int) {
+        if (q < 0) throw Error("AMG refresh: product entry outside the kept pattern", ERR_GENERIC);
+        cnt[q + 1]++;
+    });
+    for (size_t q = 0; q < nnz; q++) cnt[q + 1] += cnt[q];
+    off = cnt;
+    pl.resize(std::max<size_t>(off.back(), 1));
+    std::vector<int> next(off.begin(), off.end() - 1);
+    each([&](int q, int ka, int kb) { pl[next[q]++] = make_int2(ka, kb); });
+}
+
+}  // namespace
+
+// the symbolic structure on the device, from the host hierarchy's patterns
+static void refresh_setup(RefreshData &rf, const std::vector<HostAmgLevel> &host, hipStream_t s) {
+    const int L = (int)host.size();
+    rf.lv.clear();
+    rf.lv.resize(L);
+    for (int l = 0; l < L; l++) {
+        const HostAmgLevel &H = host[l];
+        RefreshLevel &R = rf.lv[l];
+        R.n = H.A.n;
+        upload_v(R.Arp, H.A.rp, s);
+        upload_v(R.Aci, H.A.ci, s);
+        R.Av.alloc(std::max<size_t>(H.A.ci.size(), 1));
+        R.d.alloc(std::max(H.A.n, 1));
+        if (l + 1 == L) continue;
+        upload_v(R.Prp, H.P.rp, s);
+        upload_v(R.Pci, H.P.ci, s);
+        upload_v(R.Prow, slot_rows(H.P.rp), s);
+        upload_v(R.Rrp, H.R.rp, s);
+        upload_v(R.Rci, H.R.ci, s);
+        upload_v(R.agg, H.agg, s);
+        std::vector<int> size(H.P.ncols, 0);
+        for (int a : H.agg) size[a]++;
+        std::vector<double> t(H.A.n);
+        for (int i = 0; i < H.A.n; i++) t[i] = 1.0 / std::sqrt((double)size[H.agg[i]]);
+        upload_v(R.tval, t, s);
+        // R = P^T: the P entry behind each R entry, in the host transpose's order
+        std::vector<int> next(H.R.rp.begin(), H.R.rp.end() - 1), perm(H.P.ci.size());
+        for (int i = 0; i < H.P.n; i++)
+            for (int k = H.P.rp[i]; k < H.P.rp[i + 1]; k++) perm[next[H.P.ci[k]]++] = k;
+        upload_v(R.perm, perm, s);
+        std::vector<int> aprp, apci, off;
+        std::vector<int2> pl;
+        multiply_pattern(H.A, H.P, aprp, apci);
+        R.nnzAP = (int)apci.size();
+        product_plan(H.A, H.P, aprp, apci, off, pl);
+        upload_v(R.APoff, off, s);
+        upload_v(R.APpl, pl, s);
+        HCsr AP;  // the pattern of A P, as the right operand of R (A P)
+        AP.n = H.A.n;
+        AP.ncols = H.P.ncols;
+        AP.rp = aprp;
+        AP.ci = apci;
+        const HCsr &Ac = host[l + 1].A;
+        R.nnzC = (int)Ac.ci.size();
+        product_plan(H.R, AP, Ac.rp, Ac.ci, off, pl);
+        upload_v(R.Coff, off, s);
+        upload_v(R.Cpl, pl, s);
+        R.Pv.alloc(std::max<size_t>(H.P.ci.size(), 1));
+        R.Rv.alloc(std::max<size_t>(H.R.ci.size(), 1));
+        R.APv.alloc(std::max<size_t>(apci.size(), 1));
+    }
+    const int n0 = host[0].A.n;
+    rf.rowsum.alloc(std::max(n0, 1));
+    rf.part.alloc(4096);
+    rf.w.alloc(1);
+    rf.flag.alloc(2);
+    rf.ready = true;
+}
+
+template <class V, class X>
+static void refresh_levels(AmgDev<V, X> &dv, RefreshData &rf, const std::vector<HostAmgLevel> &host, bool scale0,
+                           hipStream_t s) {
+    const int L = (int)host.size();
+    for (int l = 0; l + 1 < L; l++) {
+        RefreshLevel &R = rf.lv[l], &C = rf.lv[l + 1];
+        const HostAmgLevel &H = host[l];
+        const int n = R.n;
+        hipLaunchKernelGGL(k_rf_diag, dim3(rf_blocks(n)), dim3(BS), 0, s, n, R.Arp.p(), R.Aci.p(), R.Av.p(), R.d.p(),
+                           rf.rowsum.p());
+        const int nb = std::min(rf_blocks(n), 4096);
+        hipLaunchKernelGGL(k_rf_max, dim3(nb), dim3(BS), 0, s, n, rf.rowsum.p(), rf.part.p());
+        hipLaunchKernelGGL(k_rf_omega, dim3(1), dim3(BS), 0, s, nb, rf.part.p(), rf.w.p());
+        const int nnzP0 = (int)H.P.ci.size();
+        hipLaunchKernelGGL(k_rf_p, dim3(rf_blocks(nnzP0)), dim3(BS), 0, s, nnzP0, R.Prow.p(), R.Pci.p(), R.Arp.p(),
+                           R.Aci.p(), R.Av.p(), R.agg.p(), R.tval.p(), R.d.p(), rf.w.p(), R.Pv.p());
+        const int nnzR = (int)H.R.ci.size();
+        hipLaunchKernelGGL(k_rf_r, dim3(rf_blocks(nnzR)), dim3(BS), 0, s, nnzR, R.perm.p(), R.Pv.p(), R.Rci.p(),
+                           (scale0 && l == 0) ? R.d.p() : nullptr, R.Rv.p());
+        hipLaunchKernelGGL(k_rf_plan, dim3(rf_blocks(R.nnzAP)), dim3(BS), 0, s, R.nnzAP, R.APoff.p(), R.APpl.p(),
+                           R.Av.p(), R.Pv.p(), R.APv.p());
+        hipLaunchKernelGGL(k_rf_plan, dim3(rf_blocks(R.nnzC)), dim3(BS), 0, s, R.nnzC, R.Coff.p(), R.Cpl.p(),
+                           R.Rv.p(), R.APv.p(), C.Av.p());
+        // the device hierarchy in its value type
+        AmgLevelDev<V, X> &D = dv.lv[l];
+        const long nnzA = (long)H.A.ci.size(), nnzP = (long)H.P.ci.size();
+        hipLaunchKernelGGL((k_rf_cast<V>), dim3(rf_blocks(nnzA)), dim3(BS), 0, s, nnzA, R.Av.p(), D.A.v.p());
+        hipLaunchKernelGGL((k_rf_dinv<V>), dim3(rf_blocks(n)), dim3(BS), 0, s, n, R.d.p(), D.dinv.p());
+        hipLaunchKernelGGL((k_rf_cast<V>), dim3(rf_blocks(nnzP)), dim3(BS), 0, s, nnzP, R.Pv.p(), D.P.v.p());
+        hipLaunchKernelGGL((k_rf_cast<V>), dim3(rf_blocks(nnzR)), dim3(BS), 0, s, (long)nnzR, R.Rv.p(), D.R.v.p());
+        CUIBM_LAUNCH_CHECK();
+    }
+    RefreshLevel &Cl = rf.lv[L - 1];
+    if (dv.nc <= RF_GJ_MAX && !dv.fine.defl) {
+        hipLaunchKernelGGL((k_rf_gj<V>), dim3(1), dim3(1024), 0, s, Cl.n, Cl.Arp.p(), Cl.Aci.p(), Cl.Av.p(),
+                           dv.coarse_inv.p(), dv.ld, rf.flag.p() + 1);
+        CUIBM_LAUNCH_CHECK();
+        return;
+    }
+    // a larger coarsest level: its dense inverse on the host, as build() forms it
+    HCsr Ac;
+    Ac.n = Cl.n;
+    Ac.ncols = Cl.n;
+    Ac.rp = host[L - 1].A.rp;
+    Ac.ci = host[L - 1].A.ci;
+    Ac.v.resize(Ac.ci.size());
+    if (!Ac.v.empty()) Cl.Av.download(Ac.v.data(), Ac.v.size(), s);
+    CUIBM_HIP(hipStreamSynchronize(s));
+    std::vector<double> inv = dense_inverse(Ac);
+    if (dv.fine.defl) AmgDev<V, X>::deflate_coarse(inv, dv.nc);
+    std::vector<V> invt((size_t)dv.nc * dv.ld, V(0));
+    for (int i = 0; i < dv.nc; i++)
+        for (int j = 0; j < dv.nc; j++) invt[(size_t)i * dv.ld + j] = (V)inv[(size_t)i * dv.nc + j];
+    dv.coarse_inv.upload(invt, s);
+    CUIBM_HIP(hipStreamSynchronize(s));
+}
+
+bool AmgPrecond::refresh(const DCsr<double> &A, hipStream_t s) {
+    if (fine_) return false;  // a stencil finest level is rebuilt from its operator instead
+    RefreshData &rf = im_->rf;
+    if (!rf.ready || A.n != rf.lv[0].n) return false;
+    launch_fill_i(rf.flag.p(), 0, 2, s);
+    hipLaunchKernelGGL(k_rf_scatter, dim3(rf_blocks(A.n)), dim3(BS), 0, s, A.n, A.rp.p(), A.ci.p(), A.v.p(),
+                       rf.lv[0].Arp.p(), rf.lv[0].Aci.p(), rf.lv[0].Av.p(), rf.flag.p());
+    CUIBM_LAUNCH_CHECK();
+    int flags[2] = {0, 0};
+    rf.flag.download(flags, 1, s);
+    if (flags[0]) return false;
+    if (prm_.precision == 32) refresh_levels(im_->d32, rf, im_->host, prm_.scale0, s);
+    else if (prm_.precision == 3264) refresh_levels(im_->d3264, rf, im_->host, prm_.scale0, s);
+    else refresh_levels(im_->d64, rf, im_->host, prm_.scale0, s);
+    rf.flag.download(flags, 2, s);
+    if (flags[1]) throw Error("singular coarsest AMG operator", ERR_GENERIC);
+    return true;
 }
 
 // ---- lockstep batches: the V-cycles of many cases as one preconditioner ----

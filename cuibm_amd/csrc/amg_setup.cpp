@@ -18,6 +18,8 @@
 // Everything is sequential and in index order, so the hierarchy is deterministic.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <utility>
 
 #include "krylov.hpp"
 
@@ -205,6 +207,7 @@ std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_leve
         }
         L.A = A;
         L.P = P;
+        L.agg = agg;
         lv.push_back(std::move(L));
         A = std::move(Ac);
     }
@@ -212,6 +215,102 @@ std::vector<HostAmgLevel> amg_build(const HCsr &A0, int coarse_max, int max_leve
     last.A = A;
     lv.push_back(std::move(last));
     return lv;
+}
+
+void multiply_pattern(const HCsr &A, const HCsr &B, std::vector<int> &rp, std::vector<int> &ci) {
+    rp.assign(A.n + 1, 0);
+    ci.clear();
+    std::vector<int> seen(B.ncols, -1), cols;
+    for (int i = 0; i < A.n; i++) {
+        cols.clear();
+        for (int ka = A.rp[i]; ka < A.rp[i + 1]; ka++) {
+            const int k = A.ci[ka];
+            for (int kb = B.rp[k]; kb < B.rp[k + 1]; kb++) {
+                const int j = B.ci[kb];
+                if (seen[j] != i) {
+                    seen[j] = i;
+                    cols.push_back(j);
+                }
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        ci.insert(ci.end(), cols.begin(), cols.end());
+        rp[i + 1] = (int)ci.size();
+    }
+}
+
+HCsr pad_pattern(const HCsr &A, int nx, int margin) {
+    const int n = A.n;
+    if (nx <= 0 || n % nx != 0 || margin < 0) return A;
+    const int ny = n / nx;
+    // rows that are not 5-point rows, and the largest offset of their columns
+    std::vector<char> band(n, 0);
+    int reach = 0;
+    for (int i = 0; i < n; i++) {
+        const int ix = i % nx, iy = i / nx;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) {
+            const int dx = std::abs(A.ci[k] % nx - ix), dy = std::abs(A.ci[k] / nx - iy);
+            if (dx + dy <= 1) continue;
+            band[i] = 1;
+            reach = std::max(reach, std::max(dx, dy));
+        }
+    }
+    if (reach == 0 || reach > 4) return A;  // no such rows, or not a local stencil
+    // dilate by `margin` cells along x, then along y
+    std::vector<char> tmp(n, 0);
+    for (int iy = 0; iy < ny; iy++) {
+        int last = -(margin + 1);
+        for (int ix = 0; ix < nx; ix++) {
+            if (band[iy * nx + ix]) last = ix;
+            if (ix - last <= margin) tmp[iy * nx + ix] = 1;
+        }
+        last = nx + margin + 1;
+        for (int ix = nx - 1; ix >= 0; ix--) {
+            if (band[iy * nx + ix]) last = ix;
+            if (last - ix <= margin) tmp[iy * nx + ix] = 1;
+        }
+    }
+    std::fill(band.begin(), band.end(), 0);
+    for (int ix = 0; ix < nx; ix++) {
+        int last = -(margin + 1);
+        for (int iy = 0; iy < ny; iy++) {
+            if (tmp[iy * nx + ix]) last = iy;
+            if (iy - last <= margin) band[iy * nx + ix] = 1;
+        }
+        last = ny + margin + 1;
+        for (int iy = ny - 1; iy >= 0; iy--) {
+            if (tmp[iy * nx + ix]) last = iy;
+            if (last - iy <= margin) band[iy * nx + ix] = 1;
+        }
+    }
+    HCsr S;
+    S.n = n;
+    S.ncols = A.ncols;
+    S.rp.assign(n + 1, 0);
+    std::vector<std::pair<int, double>> row;
+    for (int i = 0; i < n; i++) {
+        row.clear();
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) row.push_back({A.ci[k], A.v[k]});
+        if (band[i]) {
+            const int ix = i % nx, iy = i / nx;
+            for (int y = std::max(0, iy - reach); y <= std::min(ny - 1, iy + reach); y++)
+                for (int x = std::max(0, ix - reach); x <= std::min(nx - 1, ix + reach); x++)
+                    row.push_back({y * nx + x, 0.0});
+        }
+        // stable: a column's own entries (in row order) come before the added zero
+        std::stable_sort(row.begin(), row.end(),
+                         [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; });
+        for (size_t q = 0; q < row.size(); q++) {
+            if (q > 0 && row[q].first == row[q - 1].first) {
+                S.v.back() += row[q].second;
+                continue;
+            }
+            S.ci.push_back(row[q].first);
+            S.v.push_back(row[q].second);
+        }
+        S.rp[i + 1] = (int)S.ci.size();
+    }
+    return S;
 }
 
 std::vector<double> dense_inverse(const HCsr &A) {

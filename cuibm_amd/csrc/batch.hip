@@ -60,7 +60,7 @@ void cat_stencils(const std::vector<const Stencils *> &src, const std::vector<lo
         node.insert(node.end(), nd.begin(), nd.end());
         idx.insert(idx.end(), ix.begin(), ix.end());
         coef.insert(coef.end(), cf.begin(), cf.end());
-        const long pn = (long)(S.nrm.n() / S.n), pe = (long)(S.extra.n() / S.n);
+        const long pn = S.nrm_per, pe = S.extra_per;
         if ((per_nrm >= 0 && pn != per_nrm) || (per_extra >= 0 && pe != per_extra))
             throw Error("lockstep batch: interpolation stencils of different layouts", ERR_ARGUMENT);
         per_nrm = pn;

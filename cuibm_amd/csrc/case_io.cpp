@@ -307,6 +307,12 @@ void readInputs(const std::string &caseFolder, int argc, const char *const *argv
             DB.PoissonSolve.amgCoarseMax = m;
             i++;
         }
+        else if (a == "-amgRefresh") {
+            const std::string o = v;
+            if (o != "0" && o != "1") throw YamlError("-amgRefresh takes 0 or 1, not '" + o + "'");
+            DB.PoissonSolve.amgRefresh = o == "1";
+            i++;
+        }
         else if (a == "-poissonPreconditioner") { DB.PoissonSolve.preconditioner = preconditionerFromString(v); i++; }
         else if (a == "-operators") {
             const std::string o = v;

@@ -36,6 +36,9 @@ struct LinearSolverParams {
     int amgPrecision = 64;
     // largest coarsest AMG level (a dense inverse); 0: the solver's default
     int amgCoarseMax = 0;
+    // a moving body's hierarchy whose matrix kept its pattern: set up again numerically on
+    // the device (true) or built anew on the host (false)
+    bool amgRefresh = true;
 };
 
 // typed restatement of the parameterDB filled by src/io/io.cu:145-197 and the

@@ -45,22 +45,34 @@ class DArray {
     explicit DArray(size_t n) { alloc(n); }
     DArray(const DArray &) = delete;
     DArray &operator=(const DArray &) = delete;
-    DArray(DArray &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DArray(DArray &&o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_) { o.p_ = nullptr; o.n_ = o.cap_ = 0; }
     DArray &operator=(DArray &&o) noexcept {
-        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            n_ = o.n_;
+            cap_ = o.cap_;
+            o.p_ = nullptr;
+            o.n_ = o.cap_ = 0;
+        }
         return *this;
     }
     ~DArray() { release(); }
+    // n elements; an allocation that already holds n is kept (arrays resized every step,
+    // such as a moving body's matrices and node lists, allocate only when they grow)
     void alloc(size_t n) {
-        if (n == n_ && p_) return;
+        if (p_ && n <= cap_) {
+            n_ = n;
+            return;
+        }
         release();
-        n_ = n;
         if (n) CUIBM_HIP(hipMalloc((void **)&p_, n * sizeof(T)));
+        n_ = cap_ = n;
     }
     void release() {
         if (p_) (void)hipFree(p_);
         p_ = nullptr;
-        n_ = 0;
+        n_ = cap_ = 0;
     }
     void upload(const T *h, size_t n, hipStream_t s) {
         alloc(n);
@@ -83,7 +95,7 @@ class DArray {
 
   private:
     T *p_ = nullptr;
-    size_t n_ = 0;
+    size_t n_ = 0, cap_ = 0;
 };
 
 // device-side CSR matrix; values in V (double or float)

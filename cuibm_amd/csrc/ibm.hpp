@@ -58,6 +58,7 @@ void launch_zero_rows(int *tags, const BoxDev &box, int rowlen, int kmax, int of
 // or, where idx_l < 0, a boundary value (body velocity / Neumann datum)
 struct Stencils {
     int n = 0;
+    int nrm_per = 0, extra_per = 0;  // values of nrm and extra per node (arrays may be longer)
     DArray<int> node, idx;        // node[n], idx[4n]
     DArray<int> idx0;             // idx before the boundary-corner substitution (4n)
     DArray<double> coef;          // coef[4n]
@@ -66,6 +67,19 @@ struct Stencils {
     DArray<double> tmp;           // staging for the ghost-node velocity update
     DArray<double> ub;            // per-node body velocity at the node's intercept (empty: uB[0] for all)
 };
+
+// The ordered node lists of one tagging, built on the device (two passes over fixed chunks
+// of candidates: counts, then in-order writes): [0] ghost and [1] hybrid velocity nodes
+// (tag > 0), [2] ghost and [3] hybrid pressure nodes, [4] solid velocity nodes (ghost tag
+// 0) in setInsideVelocity's order (u node i, then the v node J nx + I + numU of the same
+// I, J = i / (nx - 1)). out[k] must hold every candidate of list k; counts[5] receives
+// the lengths; part holds node_list_parts(...) ints.
+struct NodeListIn {
+    const int *ghostUV, *hybridUV, *ghostP, *hybridP;
+    int numU, numUV, numP, nx;
+};
+int node_list_parts(int numUV, int numP, int numU);
+void launch_node_lists(const NodeListIn &in, int *const out[5], int *part, int *counts, hipStream_t s);
 
 struct IbmGeom {
     const int *ghostUV, *hybridUV, *ghostP, *hybridP;

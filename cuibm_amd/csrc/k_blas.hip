@@ -67,6 +67,11 @@ void launch_fill_f(float *x, float v, long n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(BS), 0, s, x, v, n);
     CUIBM_LAUNCH_CHECK();
 }
+void launch_fill_i(int *x, int v, long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(n)), dim3(BS), 0, s, x, v, n);
+    CUIBM_LAUNCH_CHECK();
+}
 void launch_copy(double *y, const double *x, long n, hipStream_t s, const double *stop) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(BS), 0, s, y, x, n, stop);

@@ -912,6 +912,100 @@ void launch_tag_pressure_resolve(const GridDev &g, const BoxDev &box, const Body
     CUIBM_LAUNCH_CHECK();
 }
 
+namespace {
+
+constexpr int NL_ITEMS = 8, NL_CHUNK = BS * NL_ITEMS;
+
+// candidate c of list k: its node index, or -1 when it is not on the list
+__device__ inline int nl_pick(int k, int c, const NodeListIn &in) {
+    switch (k) {
+        case 0: return c < in.numUV && in.ghostUV[c] > 0 ? c : -1;
+        case 1: return c < in.numUV && in.hybridUV[c] > 0 ? c : -1;
+        case 2: return c < in.numP && in.ghostP[c] > 0 ? c : -1;
+        case 3: return c < in.numP && in.hybridP[c] > 0 ? c : -1;
+        default: {
+            if (c >= 2 * in.numU) return -1;
+            const int i = c >> 1;
+            if ((c & 1) == 0) return in.ghostUV[i] == 0 ? i : -1;
+            const int iv = (i / (in.nx - 1)) * in.nx + i % (in.nx - 1) + in.numU;
+            return iv < in.numUV && in.ghostUV[iv] == 0 ? iv : -1;
+        }
+    }
+}
+
+// pass 1: members of list blockIdx.y in chunk blockIdx.x
+__global__ void __launch_bounds__(BS) k_nl_count(NodeListIn in, int *__restrict__ part) {
+    __shared__ int sh[BS];
+    const int k = blockIdx.y, c0 = blockIdx.x * NL_CHUNK + threadIdx.x * NL_ITEMS;
+    int m = 0;
+    for (int e = 0; e < NL_ITEMS; e++) m += nl_pick(k, c0 + e, in) >= 0;
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// pass 2: the chunk's members written in candidate order after those of earlier chunks
+__global__ void __launch_bounds__(BS) k_nl_write(NodeListIn in, const int *__restrict__ part, int *o0, int *o1, int *o2,
+                                                 int *o3, int *o4, int *__restrict__ counts) {
+    __shared__ int sh[BS];
+    __shared__ int base_s;
+    const int k = blockIdx.y, nb = gridDim.x, c0 = blockIdx.x * NL_CHUNK + threadIdx.x * NL_ITEMS;
+    int *out = k == 0 ? o0 : k == 1 ? o1 : k == 2 ? o2 : k == 3 ? o3 : o4;
+    // members of the earlier chunks of this list
+    int b = 0;
+    for (int q = threadIdx.x; q < (int)blockIdx.x; q += BS) b += part[k * nb + q];
+    sh[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) base_s = sh[0];
+    __syncthreads();
+    const int base = base_s;
+    int m = 0;
+    for (int e = 0; e < NL_ITEMS; e++) m += nl_pick(k, c0 + e, in) >= 0;
+    // exclusive scan of the per-thread counts
+    __syncthreads();
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 1; o < BS; o <<= 1) {
+        const int v = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int at = base + sh[threadIdx.x] - m;
+    for (int e = 0; e < NL_ITEMS; e++) {
+        const int v = nl_pick(k, c0 + e, in);
+        if (v >= 0) out[at++] = v;
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == BS - 1) counts[k] = base + sh[BS - 1];
+}
+
+int nl_chunks(int numUV, int numP, int numU) {
+    const long m = std::max<long>(std::max<long>(numUV, numP), 2L * numU);
+    return (int)std::max<long>(1, (m + NL_CHUNK - 1) / NL_CHUNK);
+}
+
+}  // namespace
+
+int node_list_parts(int numUV, int numP, int numU) { return 5 * nl_chunks(numUV, numP, numU); }
+
+void launch_node_lists(const NodeListIn &in, int *const out[5], int *part, int *counts, hipStream_t s) {
+    if (in.nx < 2) throw Error("node lists: grid narrower than two cells", ERR_ARGUMENT);
+    const int nb = nl_chunks(in.numUV, in.numP, in.numU);
+    hipLaunchKernelGGL(k_nl_count, dim3(nb, 5), dim3(BS), 0, s, in, part);
+    CUIBM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nl_write, dim3(nb, 5), dim3(BS), 0, s, in, part, out[0], out[1], out[2], out[3], out[4],
+                       counts);
+    CUIBM_LAUNCH_CHECK();
+}
+
 void launch_zero_rows(int *tags, const BoxDev &box, int rowlen, int kmax, int off, int nrows, long len,
                       hipStream_t s) {
     // rows j_start .. j_end (inclusive) below nrows, and no more than the reference's thread

@@ -30,6 +30,11 @@ JacobiPrecond::JacobiPrecond(const DCsr<double> &A, double *dinv, hipStream_t s)
     launch_diag_inv(A, dinv_, s);
 }
 
+void JacobiPrecond::refresh(const DCsr<double> &A, hipStream_t s) {
+    if (A.n != n_) throw Error("Jacobi refresh: matrix size changed", ERR_ARGUMENT);
+    launch_diag_inv(A, dinv_, s);
+}
+
 void JacobiPrecond::apply(const double *r, double *z, hipStream_t s) {
     launch_jacobi_apply(n_, dinv_, r, z, s, stop_);
 }

@@ -23,6 +23,7 @@ enum { FIN_INIT = 0, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_BETA, FIN_DOT2 };
 // ---- k_blas.hip
 void launch_fill(double *x, double v, long n, hipStream_t s);
 void launch_fill_f(float *x, float v, long n, hipStream_t s);
+void launch_fill_i(int *x, int v, long n, hipStream_t s);
 void launch_copy(double *y, const double *x, long n, hipStream_t s, const double *stop = nullptr);
 void launch_to_float(float *y, const double *x, long n, hipStream_t s);
 void launch_to_double(double *y, const float *x, long n, hipStream_t s);
@@ -117,6 +118,8 @@ class JacobiPrecond : public Preconditioner {
     // D^-1 kept in caller-owned device memory of A.n doubles (allocates nothing)
     JacobiPrecond(const DCsr<double> &A, double *dinv, hipStream_t s);
     void apply(const double *r, double *z, hipStream_t s) override;
+    // D^-1 of a new matrix of the same size, in place
+    void refresh(const DCsr<double> &A, hipStream_t s);
     std::string name() const override { return "DIAGONAL"; }
     double bytes_per_apply() const override { return 24.0 * n_; }
 
@@ -142,6 +145,14 @@ struct AmgParams {
     // build G = [R | -R A] and restrict residuals in one product (an extra SpGEMM in the
     // set-up: left out for hierarchies rebuilt every step)
     bool fused_restriction = true;
+    // keep the symbolic structure on the device for refresh() (taken only with a CSR
+    // finest level, no fused restriction and at least two levels)
+    bool refreshable = false;
+    // with refreshable: the unknowns are the cells of a pad_nx-wide grid, and the
+    // hierarchy is built on pad_pattern(A, pad_nx, pad_margin), so that matrices whose
+    // body rows moved keep to its pattern (0: the pattern of A itself)
+    int pad_nx = 0;
+    int pad_margin = 8;
 };
 
 // HIP events around the launches of one kernel inside the live solve: mark() before
@@ -250,6 +261,18 @@ class AmgPrecond : public Preconditioner {
     // the stencil finest level as the V-cycle sees it (FP32 D^-1, deflation constants),
     // and 1 / w^T a of the deflation (0 without deflation)
     const P5Dev &fine_dev() const { return fine_dev_; }
+    // Numeric set-up on the device for a new matrix whose entries all lie in the pattern
+    // the hierarchy was built on (A's own, or its padded superset): A is scattered into
+    // that pattern (zeros elsewhere), the aggregates and every level's pattern are kept,
+    // and D, the prolongator smoothing, P, R (P^T, or P^T D^-1 on a row-scaled finest
+    // level), A P, R A P and the coarsest inverse (Gauss-Jordan with partial pivoting in
+    // one workgroup, up to 64 unknowns; on the host above that) are recomputed on the
+    // device, each value by the host set-up's sequence of operations. With unchanged
+    // aggregates the hierarchy is the one the host would build from the same aggregates;
+    // a fresh host set-up may aggregate differently where entries became zero or moved.
+    // Returns false when an entry of A lies outside the pattern (the caller then builds
+    // a new hierarchy). CSR finest level only.
+    bool refresh(const DCsr<double> &A, hipStream_t s);
     double deflation_inv_wa() const;
     int precision() const { return prm_.precision; }
 
@@ -265,6 +288,7 @@ class AmgPrecond : public Preconditioner {
 // host smoothed-aggregation set-up (amg_setup.cpp)
 struct HostAmgLevel {
     HCsr A, P, R;
+    std::vector<int> agg;  // the aggregate of each row (every level but the coarsest)
     // G = [R | -R A] (A.n + A.n columns): the restricted residual R (b - A x) as one
     // product with the pair (b, x), on every level but the coarsest
     HCsr G;
@@ -272,6 +296,14 @@ struct HostAmgLevel {
 std::vector<HostAmgLevel> amg_build(const HCsr &A, int coarse_max, int max_levels, bool scale0 = false,
                                     bool build_g = true);
 std::vector<double> dense_inverse(const HCsr &A);
+// A (the unknowns being the cells of an nx-wide grid) with explicit zero entries added:
+// every row within `margin` cells of a row that is not a 5-point row gets all the columns
+// within `reach` cells of it, reach being the largest column offset of those rows, so a
+// body's rows that move by up to about `margin` cells keep to this pattern. A is returned
+// unchanged when every row is a 5-point row or a column lies more than 4 cells away.
+HCsr pad_pattern(const HCsr &A, int nx, int margin);
+// the sparsity pattern (row pointers, sorted columns) of A B, as amg_build's products form it
+void multiply_pattern(const HCsr &A, const HCsr &B, std::vector<int> &rp, std::vector<int> &ci);
 
 // ---- BiCGSTAB (krylov.hip)
 struct SolveStat {

@@ -154,7 +154,8 @@ static HCsr download_csr(const DCsr<double> &A, hipStream_t s) {
 
 static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p, const DCsr<double> &A, hipStream_t s,
                                                     const P5Dev *fine = nullptr, bool scale0 = false,
-                                                    int coarse_max = 2500, bool fused_restriction = true) {
+                                                    int coarse_max = 2500, bool fused_restriction = true,
+                                                    int pad_nx = 0) {
     switch (p.preconditioner) {
         case NONE: return std::unique_ptr<Preconditioner>(new IdentityPrecond(A.n));
         case DIAGONAL: return std::unique_ptr<Preconditioner>(new JacobiPrecond(A, s));
@@ -164,6 +165,8 @@ static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p,
             ap.scale0 = scale0;
             ap.coarse_max = coarse_max;
             ap.fused_restriction = fused_restriction;
+            ap.refreshable = !fused_restriction;  // rebuilt every step: keep the symbolic structure
+            ap.pad_nx = ap.refreshable ? pad_nx : 0;
             return std::unique_ptr<Preconditioner>(new AmgPrecond(download_csr(A, s), ap, s, fine));
         }
         default: throw Error("preconditioner AINV is not supported", ERR_UNSUPPORTED);
@@ -185,7 +188,7 @@ void NavierStokesSolver::buildPreconditioners() {
     }
     PC1 = make_precond(db->velocitySolve, LHS1, stream_);
     PC2 = make_precond(db->PoissonSolve, LHS2, stream_, stencilLHS2() ? &fine : nullptr, mixedScaleLHS2(),
-                       amgCoarseMax(), !amgRebuiltEachStep());
+                       amgCoarseMax(), !amgRebuiltEachStep(), amgRebuiltEachStep() ? nx : 0);
     ks1.reset(new Bicgstab(numUV));
     ks2.reset(new Bicgstab(numP));
     CUIBM_HIP(hipStreamSynchronize(stream_));
@@ -379,7 +382,7 @@ long NavierStokesSolver::setField(const std::string &n, const void *in, long cnt
 }
 
 void NavierStokesSolver::info(double *out, int n) const {
-    std::vector<double> v(46, 0.0);
+    std::vector<double> v(47, 0.0);
     v[0] = timeStep;
     v[1] = iterationCount1;
     v[2] = iterationCount2;
@@ -401,7 +404,7 @@ void NavierStokesSolver::info(double *out, int n) const {
     v[43] = cfl_I;
     v[44] = cfl_J;
     v[45] = cfl_ts;
-    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 46));
+    std::memcpy(out, v.data(), sizeof(double) * std::min(n, 47));
 }
 
 int NavierStokesSolver::phase(const std::string &n) {
@@ -673,10 +676,7 @@ void luo_base::initialise() {
 void luo_base::tagPoints() {
     hipStream_t s = stream_;
     // fills of luo_base/tagPoints.cu:26-40 (fill kernels, no memset)
-    auto ifill = [&](DArray<int> &a, int v) {
-        std::vector<int> h(a.n(), v);
-        a.upload(h, s);
-    };
+    auto ifill = [&](DArray<int> &a, int v) { launch_fill_i(a.p(), v, (long)a.n(), s); };
     ifill(ghostTagsUV, -1);
     ifill(ghostTagsP, -1);
     ifill(hybridTagsUV, -1);
@@ -707,63 +707,60 @@ void luo_base::tagPoints() {
     launch_zero_rows(ghostTagsP.p(), box, nx, nx, 0, ny, numP, s);
     launch_zero_rows(ghostTagsUV.p(), box, nx - 1, nx - 1, 0, ny, numU, s);
     launch_zero_rows(ghostTagsUV.p(), box, nx, nx - 1, numU, ny - 1, numUV, s);
-    CUIBM_HIP(hipStreamSynchronize(s));
     buildStencils();
     tagged = true;
 }
 
-// compact node lists and interpolation weights for the current tags
+// compact node lists (on the device, in index order) and interpolation weights for the
+// current tags
 void luo_base::buildStencils() {
     hipStream_t s = stream_;
-    const std::vector<int> gUV = ghostTagsUV.to_host(s), hUV = hybridTagsUV.to_host(s);
-    const std::vector<int> gP = ghostTagsP.to_host(s), hP = hybridTagsP.to_host(s);
-    std::vector<int> lg, lh, ls, lgp, lhp;
-    for (int i = 0; i < numUV; i++) {
-        if (gUV[i] > 0) lg.push_back(i);
-        if (hUV[i] > 0) lh.push_back(i);
-    }
-    // setInsideVelocity's index map (luo_base/kernels/intermediateVelocity.cu:18-31):
-    // u everywhere, v only for I < nx-1 and inside the array
-    for (int i = 0; i < numU; i++) {
-        const int I = i % (nx - 1), J = i / (nx - 1);
-        const int iv = J * nx + I + numU;
-        if (gUV[i] == 0) ls.push_back(i);
-        if (iv < numUV && gUV[iv] == 0) ls.push_back(iv);
-    }
-    for (int i = 0; i < numP; i++) {
-        if (gP[i] > 0) lgp.push_back(i);
-        if (hP[i] > 0) lhp.push_back(i);
-    }
-    auto setup = [&](Stencils &st, const std::vector<int> &list, int nextra) {
-        st.n = (int)list.size();
-        const size_t m = std::max<size_t>(list.size(), 1);
-        st.node.upload(list, s);
-        if (list.empty()) st.node.alloc(1);
-        st.idx.alloc(4 * m);
-        st.idx0.alloc(4 * m);
-        st.coef.alloc(4 * m);
-        st.nrm.alloc(8 * m);
-        st.extra.alloc(nextra * m);
-        st.tmp.alloc(m);
+    // the node arrays hold every candidate of their list; the other stencil arrays only grow
+    auto cap = [](DArray<int> &a, size_t n) {
+        if (a.n() < n) a.alloc(n);
     };
-    setup(gnV, lg, 1);
-    setup(hnV, lh, 1);
-    setup(gnP, lgp, 3);
-    setup(hnP, lhp, 1);
+    cap(gnV.node, numUV);
+    cap(hnV.node, numUV);
+    cap(gnP.node, numP);
+    cap(hnP.node, numP);
+    cap(solid, 2 * (size_t)numU);
+    nlPart.alloc(node_list_parts(numUV, numP, numU));
+    nlCount.alloc(5);
+    const NodeListIn in{ghostTagsUV.p(), hybridTagsUV.p(), ghostTagsP.p(), hybridTagsP.p(), numU, numUV, numP, nx};
+    int *const out[5] = {gnV.node.p(), hnV.node.p(), gnP.node.p(), hnP.node.p(), solid.p()};
+    launch_node_lists(in, out, nlPart.p(), nlCount.p(), s);
+    int cnt[5];
+    nlCount.download(cnt, 5, s);
+    auto setup = [&](Stencils &st, int n, int nextra) {
+        st.n = n;
+        const size_t m = std::max(n, 1);
+        auto grow = [](auto &a, size_t want) {
+            if (a.n() < want) a.alloc(want + want / 4);
+        };
+        grow(st.idx, 4 * m);
+        grow(st.idx0, 4 * m);
+        grow(st.coef, 4 * m);
+        grow(st.nrm, 8 * m);
+        grow(st.extra, nextra * m);
+        grow(st.tmp, m);
+        st.nrm_per = 8;
+        st.extra_per = nextra;
+    };
+    setup(gnV, cnt[0], 1);
+    setup(hnV, cnt[1], 1);
+    setup(gnP, cnt[2], 3);
+    setup(hnP, cnt[3], 1);
     // per-point body velocities refer to the node lists of one tagging
     gnV.ub.release();
     hnV.ub.release();
     solidUB.release();
-    solid.upload(ls, s);
-    if (ls.empty()) solid.alloc(1);
-    nsolid = (int)ls.size();
+    nsolid = cnt[4];
     const IbmGeom m{ghostTagsUV.p(), hybridTagsUV.p(), ghostTagsP.p(), hybridTagsP.p(), bi_x.p(), bi_y.p(),
                     ip_x.p(), ip_y.p(), bip_x.p(), bip_y.p(), ipp_x.p(), ipp_y.p()};
     launch_gn_vel_coef(grid(), m, gnV, s);
     launch_hn_vel_coef(grid(), m, hnV, s);
     launch_hn_p_coef(grid(), m, hnP, s);
     launch_gn_p_coef(grid(), m, gnP, s);
-    CUIBM_HIP(hipStreamSynchronize(s));
 }
 
 // luo_base.cu:208-233. With strongly coupled VIV the step's phases repeat until the body
@@ -1067,8 +1064,8 @@ long luo_base::getField(const std::string &n, void *out, long cap) {
 
 void luo_base::info(double *out, int n) const {
     const_cast<luo_base *>(this)->readForces();
-    double v[46];
-    NavierStokesSolver::info(v, 46);
+    double v[47];
+    NavierStokesSolver::info(v, 47);
     v[5] = forceX;
     v[6] = forceY;
     v[7] = fxx;
@@ -1087,7 +1084,7 @@ void luo_base::info(double *out, int n) const {
     v[25] = nsolid;
     v[40] = scPasses;
     v[41] = centerV;
-    std::memcpy(out, v, sizeof(double) * std::min(n, 46));
+    std::memcpy(out, v, sizeof(double) * std::min(n, 47));
 }
 
 int luo_base::phase(const std::string &n) {
@@ -1206,9 +1203,9 @@ EmbAsm luoIter::asm1() const {
     a.hyb = EmbStencil{hnV.idx0.p(), hnV.idx.p(), hnV.coef.p(), hnV.nrm.p(), hnV.extra.p()};
     a.gho = EmbStencil{gnV.idx0.p(), gnV.idx.p(), gnV.coef.p(), gnV.nrm.p(), gnV.extra.p()};
     a.ghost = ghostTagsUV.p();
-    a.prp = LHS1.rp.p();
-    a.pci = LHS1.ci.p();
-    a.pv = LHS1.v.p();
+    a.prp = plain1.rp.p();
+    a.pci = plain1.ci.p();
+    a.pv = plain1.v.p();
     a.cnt = cnt.p();
     a.bc = rowc.p();
     a.bv = rowv.p();
@@ -1224,46 +1221,60 @@ EmbAsm luoIter::asm2() const {
     a.hyb = EmbStencil{hnP.idx0.p(), hnP.idx.p(), hnP.coef.p(), hnP.nrm.p(), hnP.extra.p()};
     a.gho = EmbStencil{gnP.idx0.p(), gnP.idx.p(), gnP.coef.p(), gnP.nrm.p(), gnP.extra.p()};
     a.ghost = ghostTagsP.p();
-    a.prp = LHS2.rp.p();
-    a.pci = LHS2.ci.p();
-    a.pv = LHS2.v.p();
-    a.cnt = cnt.p();
-    a.bc = rowc.p();
-    a.bv = rowv.p();
+    a.prp = plain2.rp.p();
+    a.pci = plain2.ci.p();
+    a.pv = plain2.v.p();
+    a.cnt = cnt2.p();
+    a.bc = rowc2.p();
+    a.bv = rowv2.p();
     a.ns = ns2.p();
     a.stats = stats.p() + 4;
     return a;
 }
 
-// row buffers -> CSR: the row counts come back to the host for the row pointers
-void luoIter::assemble(DCsr<double> &A, const EmbAsm &a, bool velocity, int statOff) {
-    if (velocity) launch_emb_lhs1(grid(), nu, dt, a, stream_);
-    else launch_emb_lhs2(grid(), dt, a, stream_);
-    DCsr<double> B;
-    B.n = a.n;
-    B.ncols = a.n;
-    B.rp.alloc((size_t)a.n + 1);
-    scanPart.alloc(scan_parts(a.n));
-    launch_row_scan(cnt.p(), B.rp.p(), a.n, scanPart.p(), stream_);  // row pointers on the device
-    int nnz = 0;
-    CUIBM_HIP(hipMemcpyAsync(&nnz, B.rp.p() + a.n, sizeof(int), hipMemcpyDeviceToHost, stream_));
-    CUIBM_HIP(hipStreamSynchronize(stream_));
-    B.nnz = nnz;
-    B.ci.alloc(B.nnz);
-    B.v.alloc(B.nnz);
-    launch_emb_compact(a, B.rp.p(), B.ci.p(), B.v.p(), stream_);
-    CUIBM_HIP(hipStreamSynchronize(stream_));
-    A = std::move(B);
-    (void)statOff;
+// row buffers -> CSR for both systems: the two row counts come back to the host in one
+// read-back for the sizes of the column and value arrays
+void luoIter::assemble() {
+    hipStream_t s = stream_;
+    const EmbAsm a1 = asm1(), a2 = asm2();
+    launch_emb_lhs1(grid(), nu, dt, a1, s);
+    launch_emb_lhs2(grid(), dt, a2, s);
+    DCsr<double> *A[2] = {&LHS1, &LHS2};
+    const EmbAsm *a[2] = {&a1, &a2};
+    scanPart.alloc(scan_parts(std::max(numUV, numP)));
+    nnzOut.alloc(2);
+    for (int k = 0; k < 2; k++) {
+        A[k]->n = a[k]->n;
+        A[k]->ncols = a[k]->n;
+        A[k]->rp.alloc((size_t)a[k]->n + 1);
+        launch_row_scan(a[k]->cnt, A[k]->rp.p(), a[k]->n, scanPart.p(), s);  // row pointers on the device
+        CUIBM_HIP(hipMemcpyAsync(nnzOut.p() + k, A[k]->rp.p() + a[k]->n, sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
+    int nnz[2];
+    nnzOut.download(nnz, 2, s);
+    for (int k = 0; k < 2; k++) {
+        A[k]->nnz = nnz[k];
+        A[k]->ci.alloc(nnz[k]);
+        A[k]->v.alloc(nnz[k]);
+        launch_emb_compact(*a[k], A[k]->rp.p(), A[k]->ci.p(), A[k]->v.p(), s);
+    }
 }
 
 // intermediate_velocity_setup / poisson_setup, matrix half (luo_iter/intermediateVelocity.cu:
-// 80-176, luo_iter/poisson.cu:55-136): plain matrices for the boundary rows, row maps from
-// the node lists, the two embedded assemblies, then the preconditioners
+// 80-176, luo_iter/poisson.cu:55-136): plain matrices for the boundary rows (they depend on
+// the grid, nu and dt only: formed once), row maps from the node lists, the two embedded
+// assemblies, then the preconditioners
 void luoIter::initialiseLHS() {
     hipStream_t s = stream_;
-    generateLHS1();
-    generateLHS2();
+    if (!plainReady) {
+        generateLHS1();
+        generateLHS2();
+        plain1 = std::move(LHS1);
+        plain2 = std::move(LHS2);
+        plainReady = true;
+        stats.alloc(8);
+        launch_fill_i(stats.p(), 0, 8, s);  // anomaly counts, summed over every assembly
+    }
     // a node tagged both hybrid and ghost is a hybrid row: the reference's row kernels test
     // the hybrid tag first, and its hybrid interpolation overwrites the ghost one's records
     auto make_map = [&](DArray<int> &map, int n, const Stencils &hyb, const Stencils &gho) {
@@ -1272,19 +1283,25 @@ void luoIter::initialiseLHS() {
     };
     make_map(map1, numUV, hnV, gnV);
     make_map(map2, numP, hnP, gnP);
-    const int nmax = std::max(numUV, numP);
-    cnt.alloc(nmax);
-    rowc.alloc((size_t)EMB_ROW * nmax);
-    rowv.alloc((size_t)EMB_ROW * nmax);
+    cnt.alloc(numUV);
+    rowc.alloc((size_t)EMB_ROW * numUV);
+    rowv.alloc((size_t)EMB_ROW * numUV);
+    cnt2.alloc(numP);
+    rowc2.alloc((size_t)EMB_ROW * numP);
+    rowv2.alloc((size_t)EMB_ROW * numP);
     ns1.alloc(numUV);
     ns2.alloc(numP);
-    std::vector<int> zero(8, 0);
-    stats.upload(zero, s);
-    assemble(LHS1, asm1(), true, 0);
-    assemble(LHS2, asm2(), false, 4);
-    std::vector<int> st(8);
-    stats.download(st.data(), 8, s);
-    for (int k = 0; k < 8; k++) embStats[k] += st[k];
+    assemble();
+    // a moving body's new Poisson matrix keeps to the padded pattern its hierarchy was
+    // built on while the body stays within a few cells of where it was then: the hierarchy
+    // is set up again numerically on the device, from its kept aggregates and patterns
+    auto *amg = dynamic_cast<AmgPrecond *>(PC2.get());
+    if (moving && db->PoissonSolve.amgRefresh && amg && ks1 && ks2 && amg->refresh(LHS2, s)) {
+        if (auto *jp = dynamic_cast<JacobiPrecond *>(PC1.get())) jp->refresh(LHS1, s);
+        else PC1 = make_precond(db->velocitySolve, LHS1, s);
+        refreshCount++;
+        return;
+    }
     buildPreconditioners();
 }
 
@@ -1329,10 +1346,13 @@ int luoIter::phase(const std::string &n) {
 }
 
 void luoIter::info(double *out, int n) const {
-    double v[46];
-    luo_base::info(v, 46);
-    for (int k = 0; k < 8; k++) v[32 + k] = (double)embStats[k];
-    std::memcpy(out, v, sizeof(double) * std::min(n, 46));
+    double v[47];
+    luo_base::info(v, 47);
+    int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (stats.n() >= 8) stats.download(st, 8, stream_);
+    for (int k = 0; k < 8; k++) v[32 + k] = (double)st[k];
+    v[46] = (double)refreshCount;
+    std::memcpy(out, v, sizeof(double) * std::min(n, 47));
 }
 
 // ============================================================================ modified Fadlun

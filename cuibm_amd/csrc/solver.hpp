@@ -184,6 +184,7 @@ class luo_base : public NavierStokesSolver {
     TagRecords rec;
     Stencils gnV, hnV, hnP, gnP;
     DArray<int> solid;
+    DArray<int> nlPart, nlCount;  // node-list chunk counts and list lengths
     int nsolid = 0;
     // per-point body velocities (setBodyVelocity): the velocity at each ghost / hybrid
     // node's body intercept, linear along the body segment that holds it, and rigid
@@ -261,19 +262,24 @@ class luoIter : public luo_base {
     bool centrePin() const override { return false; }
     bool stencilLHS2() const override { return false; }  // interpolation rows
     bool mixedScaleLHS2() const override { return true; }
-    // a moving body rebuilds the hierarchy every step: a small coarsest level keeps the
-    // dense inverse cheap (2 * 300^3 flops against up to 2 * 2500^3)
+    // a moving body sets the hierarchy up again every step: a coarsest level of at most
+    // 60 unknowns is inverted in one workgroup on the device (against up to 2500)
     int amgCoarseMax() const override {
-        return db->PoissonSolve.amgCoarseMax > 0 ? db->PoissonSolve.amgCoarseMax : moving ? 300 : 2500;
+        return db->PoissonSolve.amgCoarseMax > 0 ? db->PoissonSolve.amgCoarseMax : moving ? 60 : 2500;
     }
     bool amgRebuiltEachStep() const override { return moving; }
-    // row maps (-1 plain, t hybrid slot, -(t + 2) ghost slot), row buffers, row scales
-    DArray<int> map1, map2, cnt, rowc, stats, scanPart;
-    DArray<double> rowv, ns1, ns2;
-    long embStats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long refreshCount = 0;  // Poisson hierarchies set up again on the device (kept pattern)
+    // row maps (-1 plain, t hybrid slot, -(t + 2) ghost slot), row buffers of each system,
+    // row scales, the systems' nonzero counts, and the slot-anomaly counts summed over
+    // every assembly (velocity [0..3], pressure [4..7])
+    DArray<int> map1, map2, cnt, rowc, cnt2, rowc2, stats, scanPart, nnzOut;
+    DArray<double> rowv, rowv2, ns1, ns2;
+    // the plain matrices whose rows the embedded assembly keeps away from the body
+    DCsr<double> plain1, plain2;
+    bool plainReady = false;
     EmbAsm asm1() const;
     EmbAsm asm2() const;
-    void assemble(DCsr<double> &A, const EmbAsm &a, bool velocity, int statOff);
+    void assemble();
 };
 
 // modified Fadlun direct forcing (fadlunModified.h)

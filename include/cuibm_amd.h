@@ -49,7 +49,11 @@ const char *cuibm_last_error(void);
  * -poissonTol, plus -amgPrecision 64|32|3264 (as cuibm_bicgstab's amg_precision; in a
  * solver whose Poisson operator has its stencil form, 32 also removes the pinned constant
  * mode in FP64 around the FP32 V-cycle), -amgCoarseMax N (largest coarsest AMG level, solved
- * with a dense inverse: 16..6144, default 2500), -poissonPreconditioner NAME,
+ * with a dense inverse: 16..6144, default 2500; 60 for a moving embedded body),
+ * -amgRefresh 0|1 (a moving embedded body's Poisson hierarchy is built on a pattern padded
+ * around the body's rows and, while each new matrix keeps to it, set up again numerically
+ * on the device with its aggregates kept: 1, the default; or built anew on the host every
+ * step, as the reference does: 0), -poissonPreconditioner NAME,
  * -operators csr|stencil, -outputFolder DIR and -output 0|1) and construct the solver
  * (not yet initialised). */
 int cuibm_create(const char *case_folder, int argc, const char *const *argv, cuibm_solver **out);
@@ -132,7 +136,9 @@ long cuibm_set_field(cuibm_solver *s, const char *name, const void *host, long n
  * the 3x3 stencil (the reference's result depends on stale slots when any is nonzero);
  * immersed-body solvers: [40] passes of the last step (above 1 only for strongly coupled
  * VIV), [41] transverse velocity of the body centre; [42] largest CFL number so far,
- * [43..45] its cell I, J and time step (NavierStokes/CFL.cu:10-35). n may be up to 46. */
+ * [43..45] its cell I, J and time step (NavierStokes/CFL.cu:10-35); OSC_CYLINDER with a
+ * moving body: [46] Poisson hierarchies set up again on the device (the others, the first
+ * included, were built on the host). n may be up to 47. */
 int cuibm_info(cuibm_solver *s, double *out, int n);
 
 /* Logger (src/logger.h:94-232): accumulated host wall-clock seconds per phase, with the

@@ -394,8 +394,8 @@ void Solver::embedded_pass() {
     hn_pressure();
     buildLHS2_embedded();
     // interpolation rows (unit diagonal) beside Poisson rows; a moving body rebuilds the
-    // hierarchy every step on the product side, which then stops coarsening at 300 unknowns
-    PC2.build(LHS2, P.precond2, true, B.moving ? 300 : 0);
+    // hierarchy every step on the product side, which then stops coarsening at 60 unknowns
+    PC2.build(LHS2, P.precond2, true, B.moving ? 60 : 0);
     update_rhs2_embedded();
     solve2();
     // luo_base/projectVelocity.cu:11-53

@@ -124,6 +124,14 @@ def test_amg_coarse_max_is_parsed_and_bounded():
         assert e.value.code == -2 and "16..6144" in str(e.value)
 
 
+def test_amg_refresh_switch_is_parsed():
+    for ok in ("0", "1"):
+        cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-amgRefresh", ok).close()
+    with pytest.raises(cuibm_amd.CuibmError) as e:
+        cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-amgRefresh", "yes")
+    assert e.value.code == -2 and "0 or 1" in str(e.value)
+
+
 def test_csr_create_rejects_a_malformed_row_pointer():
     """row pointers that do not start at 0 or that decrease are refused before any upload"""
     import ctypes as C

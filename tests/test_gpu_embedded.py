@@ -15,7 +15,9 @@ did, its matrix would depend on the previous step's sorted arrays).
 import numpy as np
 import pytest
 
-from helpers import pair, rel_l2
+import cuibm_amd
+import pyoracle as po
+from helpers import case, pair, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +26,7 @@ EMB_KEYS = ("emb1_unwritten", "emb1_tail_unwritten", "emb1_tail_multi", "emb1_dr
             "emb2_unwritten", "emb2_tail_unwritten", "emb2_tail_multi", "emb2_dropped")
 
 
-def _compare_step(g, o, k):
+def _compare_step(g, o, k, poisson_its_rel=0.05):
     du = rel_l2(g.get("u"), o.get("u"))
     pg, po_ = g.get("p"), o.get("p")
     dp = rel_l2(pg - pg.mean(), po_ - po_.mean())
@@ -41,7 +43,7 @@ def _compare_step(g, o, k):
     # at a 1e-12..1e-13 stop the last Poisson iterations sit at the rounding floor, so
     # where BiCGSTAB stops can move by a few iterations between the two reduction orders
     # (up to 6 of 131 on the 462x462 case); the fields above are the parity measure
-    assert abs(gi["iter2"] - oi["iter2"]) <= max(3, 0.05 * oi["iter2"]), k
+    assert abs(gi["iter2"] - oi["iter2"]) <= max(3, poisson_its_rel * oi["iter2"]), k
 
 
 def _no_anomalies(g, o):
@@ -77,16 +79,26 @@ def test_embedded_steps_match_oracle(name, steps):
     _no_anomalies(g, o)
 
 
-def test_embedded_oscillating_cylinder_matches_oracle():
+@pytest.mark.parametrize("refresh", ["0", "1"])
+def test_embedded_oscillating_cylinder_matches_oracle(refresh):
     """In-line oscillation with the embedded method (validation/error/oscflow2/embedded05
-    of the reference): after each step's move the tags, both matrices and both
-    preconditioners are rebuilt."""
-    g, o = pair("oscFlowRe100_embedded", {"velocityTol": 1e-12, "poissonTol": 1e-13})
+    of the reference): after each step's move the tags and both matrices are formed again.
+    With -amgRefresh 0 both preconditioners are rebuilt every step, as the reference and
+    the oracle do, and the bars are those above. With the default device re-set-up the
+    Poisson hierarchy keeps its first aggregates, so its iteration counts may differ from
+    the oracle's by more than the rounding floor: within 15%; the fields keep their bars."""
+    tol = {"velocityTol": 1e-12, "poissonTol": 1e-13}
+    g = cuibm_amd.Solver(case("oscFlowRe100_embedded"), "-velocityTol", "1e-12", "-poissonTol", "1e-13",
+                         "-amgRefresh", refresh)
+    g.initialise()
+    o = po.Oracle(po.read_case(case("oscFlowRe100_embedded"), tol))
     for k in range(6):
         g.step_time()
         o.step()
         for t in TAGS:
             assert np.array_equal(g.get(t), o.get(t)), (k, t)
         assert g.info()["midX"] == o.info()["midX"], k
-        _compare_step(g, o, k)
+        _compare_step(g, o, k, poisson_its_rel=0.05 if refresh == "0" else 0.15)
     _no_anomalies(g, o)
+    assert g.info()["amg_refreshes"] == (6 if refresh == "1" else 0)
+    g.close()
