@@ -237,10 +237,13 @@ PUBLISHED = [
 ]
 
 
-def published_runs(cu):
-    """each published workload run in full (nt steps from the impulsive start), FP64 hierarchy"""
+def published_runs(cu, only=None):
+    """each published workload run in full (nt steps from the impulsive start), FP64 hierarchy;
+    the per-step phase times come from the solver's own timers"""
     rows = []
     for name, what, ref_s, hw in PUBLISHED:
+        if only and name not in only:
+            continue
         t0 = time.perf_counter()
         s = cu.Solver(os.path.join(cu.CASES, name), "-amgPrecision", "64", diagnostics=False)
         s.initialise()
@@ -257,22 +260,28 @@ def published_runs(cu):
         cu.device_synchronize()
         wall = time.perf_counter() - w0
         i1 = s.info()
+        phases = {k: round(1e3 * v / nt, 4) for k, v in s.timers().items() if v > 0}
         s.close()
         run_s = t_init + wall
         rows.append({"case": name, "workload": what, "steps": nt, "init_s": round(t_init, 3),
                      "steps_s": round(wall, 3), "steps_per_s": round(1e3 * nt / ms, 2), "run_s": round(run_s, 3),
                      "published_s": ref_s, "published_hardware": hw, "speedup_vs_published": round(ref_s / run_s, 1),
-                     "poisson_iters_per_step": round(i1["iter2_total"] / nt, 2), "cd": 2.0 * i1["forceX"]})
-        log(f"[bench] {name}: {nt} steps in {wall:.2f} s (+{t_init:.2f} s set-up), published {ref_s:.0f} s")
+                     "poisson_iters_per_step": round(i1["iter2_total"] / nt, 2), "cd": 2.0 * i1["forceX"],
+                     "amg_refreshes": int(i1["amg_refreshes"]), "phase_ms_per_step": phases})
+        log(f"[bench] {name}: {nt} steps in {wall:.2f} s (+{t_init:.2f} s set-up), published {ref_s:.0f} s; "
+            f"Update Solver {phases.get('Update Solver', 0.0):.3f} ms/step, Poisson Solve "
+            f"{phases.get('Poisson Solve', 0.0):.3f} ms/step")
     return rows
 
 
-def adjoint_bench(cu):
-    """one objective + gradient of the rotating cylinder (cuibm_amd.adjoint)"""
+def adjoint_bench(cu, nsteps=800):
+    """one objective + gradient of the rotating cylinder (cuibm_amd.adjoint); J over the
+    second half of the nsteps steps"""
     import numpy as np
     from cuibm_amd.adjoint import RotatingCylinder
 
-    p = RotatingCylinder(os.path.join(cu.CASES, "rotatingCylinderRe100_luo"), nsteps=800, n0=400, n1=800, every=20)
+    p = RotatingCylinder(os.path.join(cu.CASES, "rotatingCylinderRe100_luo"), nsteps=nsteps, n0=nsteps // 2,
+                         n1=nsteps, every=20)
     blow = np.random.default_rng(2024).normal(0.0, 0.01, p.npts)
     t0 = time.perf_counter()
     J, ga, gb = p.evaluate(0.5, blow)
@@ -291,14 +300,14 @@ def adjoint_bench(cu):
         "J": J,
         "dJ_dalpha": ga,
         "dJ_dblow_norm": float(np.linalg.norm(gb)),
-        "workload": "rotatingCylinderRe100_luo: 498x498, 256 points, alpha=0.5 + seeded blowing, 800 steps of "
-                    "dt=0.005, J = mean C_D over steps 400..800 (t = 2..4), gradient w.r.t. alpha and 256 blowing "
-                    "values, checkpoints every 20 steps, FP64, solves to 1e-12",
+        "workload": f"rotatingCylinderRe100_luo: 498x498, 256 points, alpha=0.5 + seeded blowing, {nsteps} steps "
+                    f"of dt=0.005, J = mean C_D over steps {nsteps // 2}..{nsteps}, gradient w.r.t. alpha and 256 "
+                    f"blowing values, checkpoints every 20 steps, FP64, solves to 1e-12",
         "timing": "wall clock of one call (fresh solver set-up, forward run, backward sweep)",
     }
 
 
-def ensemble_bench(cu, n, steps, workers, check, prec="3264"):
+def ensemble_bench(cu, n, steps, workers, check, prec="32", modes=("lockstep", "concurrent")):
     """n seeded cylinder cases of 96^2..512^2 cells and Re 40..200 (cuibm_amd.ensemble),
     run in lockstep (every phase one launch over all cases) and, for comparison, as
     concurrent solvers on their own streams; every (n // check)-th case is then run alone
@@ -310,15 +319,21 @@ def ensemble_bench(cu, n, steps, workers, check, prec="3264"):
     paths, cases = ens.write_cases(folder, n=n, steps=steps)
     cells = sum(c[0] ** 2 for c in cases)
     log(f"[bench] ensemble: {n} cases, {cells} cells, {steps} steps each, AMG precision {prec}")
-    r = cu.ensemble_run(paths, "-amgPrecision", prec, steps=steps, workers=workers, lockstep=True)
-    rate = cells * steps / r["steps_wall_s"]
-    log(f"[bench] lockstep: {rate / 1e6:.1f} M cell-updates/s over the steps ({r['steps_wall_s']:.2f} s), "
-        f"{r['wall_s']:.2f} s with set-up")
-    rc = cu.ensemble_run(paths, "-amgPrecision", prec, steps=steps, workers=workers)
-    rate_c = cells * steps / rc["steps_wall_s"]
-    log(f"[bench] concurrent solvers ({workers} host threads): {rate_c / 1e6:.1f} M cell-updates/s "
-        f"({rc['steps_wall_s']:.2f} s)")
-    picked = list(range(0, n, max(1, n // check)))[:check]
+    r = rc = None
+    rate = rate_c = None
+    if "lockstep" in modes:
+        r = cu.ensemble_run(paths, "-amgPrecision", prec, steps=steps, workers=workers, lockstep=True)
+        rate = cells * steps / r["steps_wall_s"]
+        log(f"[bench] lockstep: {rate / 1e6:.1f} M cell-updates/s over the steps ({r['steps_wall_s']:.2f} s), "
+            f"{r['wall_s']:.2f} s with set-up")
+    if "concurrent" in modes:
+        rc = cu.ensemble_run(paths, "-amgPrecision", prec, steps=steps, workers=workers)
+        rate_c = cells * steps / rc["steps_wall_s"]
+        log(f"[bench] concurrent solvers ({workers} host threads): {rate_c / 1e6:.1f} M cell-updates/s "
+            f"({rc['steps_wall_s']:.2f} s)")
+    if r is None:
+        r, rate = rc, rate_c
+    picked = list(range(0, n, max(1, n // check)))[:check] if check > 0 else []
     dcd, dcd_c, t_alone = [], [], 0.0
     for i in picked:
         s = cu.Solver(paths[i], "-amgPrecision", "64", diagnostics=False)
@@ -327,10 +342,11 @@ def ensemble_bench(cu, n, steps, workers, check, prec="3264"):
         t_alone += ms * 1e-3
         cd64 = 2.0 * s.info()["forceX"]
         dcd.append(abs(cd64 - r["cd"][i]))
-        dcd_c.append(abs(cd64 - rc["cd"][i]))
+        if rc is not None:
+            dcd_c.append(abs(cd64 - rc["cd"][i]))
         s.close()
         log(f"[bench]   case {i} ({cases[i][0]}^2, Re {cases[i][1]:.1f}) alone in FP64: |dC_D| {dcd[-1]:.2e}")
-    alone_rate = sum(cases[i][0] ** 2 for i in picked) * steps / t_alone
+    alone_rate = sum(cases[i][0] ** 2 for i in picked) * steps / t_alone if picked else None
     import shutil
     shutil.rmtree(folder, ignore_errors=True)
     return {
@@ -343,12 +359,13 @@ def ensemble_bench(cu, n, steps, workers, check, prec="3264"):
                                f"rtol 1e-8; lockstep: every phase one launch over all cases",
                    "cases": n, "steps": steps, "cells": cells, "workers": workers, "amg_precision": prec},
         "steps_wall_s": round(r["steps_wall_s"], 3), "wall_s_with_setup": round(r["wall_s"], 3),
-        "concurrent_cell_updates_per_s": round(rate_c, 1),
-        "concurrent_steps_wall_s": round(rc["steps_wall_s"], 3),
-        "speedup_lockstep_vs_concurrent": round(rate / rate_c, 3),
-        "alone_fp64_cell_updates_per_s": round(alone_rate, 1),
-        "check": {"cases": picked, "max_abs_dcd_vs_alone_fp64": max(dcd),
-                  "max_abs_dcd_concurrent_vs_alone_fp64": max(dcd_c)},
+        "modes": list(modes),
+        "concurrent_cell_updates_per_s": round(rate_c, 1) if rate_c else None,
+        "concurrent_steps_wall_s": round(rc["steps_wall_s"], 3) if rc else None,
+        "speedup_lockstep_vs_concurrent": round(rate / rate_c, 3) if rate_c and "lockstep" in modes else None,
+        "alone_fp64_cell_updates_per_s": round(alone_rate, 1) if alone_rate else None,
+        "check": {"cases": picked, "max_abs_dcd_vs_alone_fp64": max(dcd) if dcd else None,
+                  "max_abs_dcd_concurrent_vs_alone_fp64": max(dcd_c) if dcd_c else None},
     }
 
 
@@ -372,13 +389,20 @@ def main():
     ap.add_argument("--published", action="store_true",
                     help="run the reference's published whole-run workloads in full and print their times "
                          "beside the published ones (one JSON line; not the headline metric)")
+    ap.add_argument("--published-cases", default="",
+                    help="comma-separated case folders of --published to run (default: all five)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="run N seeded cylinder cases concurrently and print the aggregate cell-update rate "
                          "(one JSON line; not the headline)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-workers", type=int, default=16)
     ap.add_argument("--ensemble-check", type=int, default=8, help="cases re-run alone in FP64 for the C_D check")
-    ap.add_argument("--ensemble-precision", default="3264", help="AMG precision of the ensemble cases (64, 32, 3264)")
+    ap.add_argument("--ensemble-precision", default="32", help="AMG precision of the ensemble cases (64, 32, 3264)")
+    ap.add_argument("--ensemble-modes", default="lockstep,concurrent",
+                    help="comma list of ensemble forms to run: lockstep, concurrent")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="time one discrete-adjoint gradient only (one JSON line; not the headline)")
+    ap.add_argument("--adjoint-steps", type=int, default=800, help="time steps of the adjoint workload")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -392,10 +416,15 @@ def main():
     case = os.path.join(cu.CASES, CASE)
     if args.ensemble > 0:
         print(json.dumps(ensemble_bench(cu, args.ensemble, args.ensemble_steps, args.ensemble_workers,
-                                        args.ensemble_check, args.ensemble_precision)), flush=True)
+                                        args.ensemble_check, args.ensemble_precision,
+                                        tuple(args.ensemble_modes.split(",")))), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"metric": "seconds per discrete-adjoint gradient", "unit": "s", "higher_is_better": False,
+                          "n_gpus": 1, "dtype": "f64", "adjoint": adjoint_bench(cu, args.adjoint_steps)}), flush=True)
         return
     if args.published:
-        rows = published_runs(cu)
+        rows = published_runs(cu, args.published_cases.split(",") if args.published_cases else None)
         print(json.dumps({"metric": "seconds per published whole run (set-up + all steps, no file output)",
                           "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
                           "runs": rows}), flush=True)
@@ -431,7 +460,7 @@ def main():
     adj = None
     if not args.no_adjoint:
         log("[bench] discrete adjoint of the rotating cylinder (498x498, 800 steps)")
-        adj = adjoint_bench(cu)
+        adj = adjoint_bench(cu, args.adjoint_steps)
     cpu = None
     if args.cpu_steps > 0:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
