@@ -307,12 +307,15 @@ def adjoint_bench(cu, nsteps=800):
     }
 
 
-def ensemble_bench(cu, n, steps, workers, check, prec="32", modes=("lockstep", "concurrent")):
+def ensemble_bench(cu, n, steps, workers, check, prec="32", modes=("lockstep", "eager", "concurrent")):
     """n seeded cylinder cases of 96^2..512^2 cells and Re 40..200 (cuibm_amd.ensemble),
-    run in lockstep (every phase one launch over all cases) and, for comparison, as
-    concurrent solvers on their own streams; every (n // check)-th case is then run alone
-    with the FP64 hierarchy and its drag coefficient compared with both"""
+    run in lockstep (every phase one launch over all cases, the solves' iteration blocks as
+    captured graphs), in lockstep with every kernel launched one by one ("eager") and, for
+    comparison, as concurrent solvers on their own streams; every (n // check)-th case is
+    then run alone with the FP64 hierarchy and its drag coefficient compared with them"""
     import tempfile
+
+    import numpy as np
     from cuibm_amd import ensemble as ens
 
     folder = tempfile.mkdtemp(prefix="cuibm_ensemble_")
@@ -326,13 +329,21 @@ def ensemble_bench(cu, n, steps, workers, check, prec="32", modes=("lockstep", "
         rate = cells * steps / r["steps_wall_s"]
         log(f"[bench] lockstep: {rate / 1e6:.1f} M cell-updates/s over the steps ({r['steps_wall_s']:.2f} s), "
             f"{r['wall_s']:.2f} s with set-up")
+    re_ = rate_e = None
+    if "eager" in modes:
+        re_ = cu.ensemble_run(paths, "-amgPrecision", prec, "-lockstepGraphs", "0", steps=steps, workers=workers,
+                              lockstep=True)
+        rate_e = cells * steps / re_["steps_wall_s"]
+        same = r is not None and np.array_equal(r["cd"], re_["cd"]) and np.array_equal(r["cl"], re_["cl"])
+        log(f"[bench] lockstep, kernels launched one by one: {rate_e / 1e6:.1f} M cell-updates/s "
+            f"({re_['steps_wall_s']:.2f} s); C_D and C_L {'equal' if same else 'differ from'} the graph run's")
     if "concurrent" in modes:
         rc = cu.ensemble_run(paths, "-amgPrecision", prec, steps=steps, workers=workers)
         rate_c = cells * steps / rc["steps_wall_s"]
         log(f"[bench] concurrent solvers ({workers} host threads): {rate_c / 1e6:.1f} M cell-updates/s "
             f"({rc['steps_wall_s']:.2f} s)")
     if r is None:
-        r, rate = rc, rate_c
+        r, rate = (re_, rate_e) if re_ is not None else (rc, rate_c)
     picked = list(range(0, n, max(1, n // check)))[:check] if check > 0 else []
     dcd, dcd_c, t_alone = [], [], 0.0
     for i in picked:
@@ -360,6 +371,9 @@ def ensemble_bench(cu, n, steps, workers, check, prec="32", modes=("lockstep", "
                    "cases": n, "steps": steps, "cells": cells, "workers": workers, "amg_precision": prec},
         "steps_wall_s": round(r["steps_wall_s"], 3), "wall_s_with_setup": round(r["wall_s"], 3),
         "modes": list(modes),
+        "eager_lockstep_cell_updates_per_s": round(rate_e, 1) if rate_e else None,
+        "graphs_equal_eager": bool(np.array_equal(r["cd"], re_["cd"]) and np.array_equal(r["cl"], re_["cl"]))
+        if re_ is not None and "lockstep" in modes else None,
         "concurrent_cell_updates_per_s": round(rate_c, 1) if rate_c else None,
         "concurrent_steps_wall_s": round(rc["steps_wall_s"], 3) if rc else None,
         "speedup_lockstep_vs_concurrent": round(rate / rate_c, 3) if rate_c and "lockstep" in modes else None,
@@ -398,8 +412,9 @@ def main():
     ap.add_argument("--ensemble-workers", type=int, default=16)
     ap.add_argument("--ensemble-check", type=int, default=8, help="cases re-run alone in FP64 for the C_D check")
     ap.add_argument("--ensemble-precision", default="32", help="AMG precision of the ensemble cases (64, 32, 3264)")
-    ap.add_argument("--ensemble-modes", default="lockstep,concurrent",
-                    help="comma list of ensemble forms to run: lockstep, concurrent")
+    ap.add_argument("--ensemble-modes", default="lockstep,eager,concurrent",
+                    help="comma list of ensemble forms to run: lockstep (iteration blocks as graphs), eager "
+                         "(lockstep, kernels launched one by one), concurrent")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time one discrete-adjoint gradient only (one JSON line; not the headline)")
     ap.add_argument("--adjoint-steps", type=int, default=800, help="time steps of the adjoint workload")

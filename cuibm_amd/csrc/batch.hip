@@ -121,6 +121,15 @@ struct LockstepBatch::Impl {
     DArray<double> fpart, fout;
     double tol1 = 0, tol2 = 0;
     int maxit1 = 0, maxit2 = 0;
+    // eight BiCGSTAB iterations of each operator as one captured graph
+    static constexpr int GRAPH_ITS = 8;
+    bool graphs = true;
+    hipGraphExec_t block[2] = {nullptr, nullptr};
+
+    ~Impl() {
+        for (hipGraphExec_t &g : block)
+            if (g) (void)hipGraphExecDestroy(g);
+    }
 
     void fetch() {
         CUIBM_HIP(hipMemcpyAsync(host_sc.data(), sc.p(), host_sc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -137,6 +146,23 @@ struct LockstepBatch::Impl {
         const BMapDev md = m.dev();
         launch_b_init(op, tab.p(), md, x, b, w.r.p(), w.p.p(), w.rs.p(), w.part.p(), s);
         launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_INIT, rtol, maxit, s);
+        // one iteration: every launch is the same in every iteration (the stop codes and the
+        // half-step state live in the per-case scalar blocks), so a block of them can be
+        // captured once and replayed
+        auto iteration = [&]() {
+            precond(w.p.p(), w.Mp.p());
+            launch_b_spmv_dot(op, tab.p(), md, w.Mp.p(), w.AMp.p(), w.rs.p(), false, w.part.p(), s);
+            launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_ALPHA, rtol, maxit, s);
+            launch_b_bicg_s(op, tab.p(), md, w.r.p(), w.AMp.p(), w.sv.p(), w.part.p(), s);
+            launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_SS, rtol, maxit, s);
+            precond(w.sv.p(), w.Ms.p());
+            launch_b_spmv_dot(op, tab.p(), md, w.Ms.p(), w.AMs.p(), w.sv.p(), true, w.part.p(), s);
+            launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_OMEGA, rtol, maxit, s);
+            launch_b_bicg_xr(op, tab.p(), md, x, w.Mp.p(), w.Ms.p(), w.r.p(), w.sv.p(), w.AMs.p(), w.rs.p(),
+                             w.part.p(), s);
+            launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_BETA, rtol, maxit, s);
+            launch_b_bicg_p(op, tab.p(), md, w.p.p(), w.r.p(), w.AMp.p(), s);
+        };
         int it = 0;
         for (;;) {
             fetch();
@@ -144,20 +170,24 @@ struct LockstepBatch::Impl {
             for (int c = 0; c < nc; c++)
                 if (scv(c, op)[SC_STOP] == 0.0) running = true;
             if (!running || it >= maxit) break;
-            const int batch = std::min(8, maxit - it);
-            for (int j = 0; j < batch; j++, it++) {
-                precond(w.p.p(), w.Mp.p());
-                launch_b_spmv_dot(op, tab.p(), md, w.Mp.p(), w.AMp.p(), w.rs.p(), false, w.part.p(), s);
-                launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_ALPHA, rtol, maxit, s);
-                launch_b_bicg_s(op, tab.p(), md, w.r.p(), w.AMp.p(), w.sv.p(), w.part.p(), s);
-                launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_SS, rtol, maxit, s);
-                precond(w.sv.p(), w.Ms.p());
-                launch_b_spmv_dot(op, tab.p(), md, w.Ms.p(), w.AMs.p(), w.sv.p(), true, w.part.p(), s);
-                launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_OMEGA, rtol, maxit, s);
-                launch_b_bicg_xr(op, tab.p(), md, x, w.Mp.p(), w.Ms.p(), w.r.p(), w.sv.p(), w.AMs.p(), w.rs.p(),
-                                 w.part.p(), it, s);
-                launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_BETA, rtol, maxit, s);
-                launch_b_bicg_p(op, tab.p(), md, w.p.p(), w.r.p(), w.AMp.p(), s);
+            const int batch = std::min(GRAPH_ITS, maxit - it);
+            if (graphs && batch == GRAPH_ITS) {
+                // the block reads and writes the same arrays every time (x and the
+                // operator's workspace): capture it at its first use
+                hipGraphExec_t &g = block[op == BOP_VEL ? 0 : 1];
+                if (!g) {
+                    hipGraph_t graph = nullptr;
+                    CUIBM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    for (int j = 0; j < GRAPH_ITS; j++) iteration();
+                    CUIBM_HIP(hipStreamEndCapture(s, &graph));
+                    const hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    CUIBM_HIP(e);
+                }
+                CUIBM_HIP(hipGraphLaunch(g, s));
+                it += GRAPH_ITS;
+            } else {
+                for (int j = 0; j < batch; j++, it++) iteration();
             }
         }
         int most = 0;
@@ -175,9 +205,11 @@ struct LockstepBatch::Impl {
     }
 };
 
-LockstepBatch::LockstepBatch(const std::vector<NavierStokesSolver *> &cases, hipStream_t s) : im_(new Impl) {
+LockstepBatch::LockstepBatch(const std::vector<NavierStokesSolver *> &cases, hipStream_t s, bool graphs)
+    : im_(new Impl) {
     Impl &B = *im_;
     B.s = s;
+    B.graphs = graphs;
     B.nc = (int)cases.size();
     if (B.nc == 0) throw Error("lockstep batch without cases", ERR_ARGUMENT);
     std::vector<luoIBM *> luo(B.nc);

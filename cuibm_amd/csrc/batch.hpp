@@ -81,7 +81,7 @@ void launch_b_spmv_dot(int op, const BCase *tab, const BMapDev &m, const double 
 void launch_b_bicg_s(int op, const BCase *tab, const BMapDev &m, const double *r, const double *AMp, double *s_,
                      double *part, hipStream_t s);
 void launch_b_bicg_xr(int op, const BCase *tab, const BMapDev &m, double *x, const double *Mp, const double *Ms,
-                      double *r, const double *s_, const double *AMs, const double *rs, double *part, int k,
+                      double *r, const double *s_, const double *AMs, const double *rs, double *part,
                       hipStream_t s);
 void launch_b_bicg_p(int op, const BCase *tab, const BMapDev &m, double *p, const double *r, const double *AMp,
                      hipStream_t s);
@@ -139,7 +139,10 @@ class NavierStokesSolver;
 // batch's arrays, stepped there, and can be copied back (copy_back).
 class LockstepBatch {
   public:
-    LockstepBatch(const std::vector<NavierStokesSolver *> &cases, hipStream_t s);
+    // graphs: run the solves' eight-iteration blocks as captured HIP graphs (one per
+    // operator, captured at its first full block and replayed after) instead of launching
+    // their kernels one by one
+    LockstepBatch(const std::vector<NavierStokesSolver *> &cases, hipStream_t s, bool graphs = true);
     ~LockstepBatch();
     void step();
     // per case: 2 Fx, 2 Fy after the last step (input order)

@@ -458,11 +458,24 @@ extern "C" int cuibm_ensemble_lockstep(int n, const char *const *case_folders, i
     };
     if (n <= 0 || !case_folders) return fail(CUIBM_ERR_ARGUMENT, "no cases");
     if (workers < 1 || workers > 64) return fail(CUIBM_ERR_ARGUMENT, "workers must be 1..64");
+    // -lockstepGraphs 0|1 is the batch's own switch; the other overrides go to every case
+    bool graphs = true;
+    std::vector<const char *> args;
+    for (int i = 0; i < argc; i++) {
+        if (argv[i] && std::string(argv[i]) == "-lockstepGraphs") {
+            const std::string v = i + 1 < argc && argv[i + 1] ? argv[i + 1] : "";
+            if (v != "0" && v != "1") return fail(CUIBM_ERR_INPUT, "-lockstepGraphs takes 0 or 1");
+            graphs = v == "1";
+            i++;
+            continue;
+        }
+        args.push_back(argv[i]);
+    }
     const double t0 = now();
     int nsteps = -1;
     for (int i = 0; i < n; i++) {
         cuibm_solver *h = nullptr;
-        const int rc = cuibm_create(case_folders[i], argc, argv, &h);
+        const int rc = cuibm_create(case_folders[i], (int)args.size(), args.data(), &h);
         if (rc != 0) return fail(rc, "case " + std::to_string(i) + ": " + cuibm_last_error());
         hs.push_back(h);
         const int k = steps > 0 ? steps : h->db.nt;
@@ -501,7 +514,7 @@ extern "C" int cuibm_ensemble_lockstep(int n, const char *const *case_folders, i
         for (auto *h : hs) CUIBM_HIP(hipStreamSynchronize(h->s->stream()));
         std::vector<NavierStokesSolver *> ss;
         for (auto *h : hs) ss.push_back(h->s.get());
-        LockstepBatch batch(ss, hs[0]->s->stream());
+        LockstepBatch batch(ss, hs[0]->s->stream(), graphs);
         t1 = now();
         for (int k = 0; k < nsteps; k++) batch.step();
         std::vector<double> vcd, vcl;

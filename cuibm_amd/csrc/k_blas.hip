@@ -205,15 +205,15 @@ void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const 
 }
 
 // x += alpha Mp + omega Ms; r = s - omega AMs; partials of (rs.r) and (r.r). In the
-// iteration k that ended at its half step (SC_STOP = STOP_HALF, SC_HALFK = k) it applies
-// the half-step update x += alpha Mp alone.
+// iteration that ended at its half step (SC_STOP = STOP_HALF_PENDING) it applies the
+// half-step update x += alpha Mp alone.
 __global__ void __launch_bounds__(BS) k_bicg_xr(int n, double *__restrict__ x, const double *__restrict__ Mp,
                                                 const double *__restrict__ Ms, double *__restrict__ r,
                                                 const double *__restrict__ s, const double *__restrict__ AMs,
                                                 const double *__restrict__ rs, const double *__restrict__ sc,
-                                                double *__restrict__ part, int k) {
+                                                double *__restrict__ part) {
     const double stopv = sc[SC_STOP], alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
-    const bool half = stopv == STOP_HALF && sc[SC_HALFK] == (double)k;
+    const bool half = stopv == STOP_HALF_PENDING;
     double acc[2] = {0, 0};
     bool tested = false;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -242,8 +242,8 @@ __global__ void __launch_bounds__(BS) k_bicg_xr(int n, double *__restrict__ x, c
 }
 
 void launch_bicg_xr(int n, double *x, const double *Mp, const double *Ms, double *r, const double *s_,
-                    const double *AMs, const double *rs, const double *sc, double *part, int k, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_xr, dim3(RED_GRID), dim3(BS), 0, s, n, x, Mp, Ms, r, s_, AMs, rs, sc, part, k);
+                    const double *AMs, const double *rs, const double *sc, double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_xr, dim3(RED_GRID), dim3(BS), 0, s, n, x, Mp, Ms, r, s_, AMs, rs, sc, part);
     CUIBM_LAUNCH_CHECK();
 }
 
@@ -281,7 +281,7 @@ __device__ __forceinline__ void finish_update(int op, const double (&tot)[2], do
             if (!isfinite(tot[0]) || !isfinite(sc[SC_ALPHA])) {
                 sc[SC_STOP] = STOP_BREAK_HALF;
             } else if (sqrt(tot[0]) <= tol) {
-                sc[SC_STOP] = STOP_HALF;
+                sc[SC_STOP] = STOP_HALF_PENDING;
                 sc[SC_HALFK] = sc[SC_ITS];
             }
             break;
@@ -317,6 +317,10 @@ __device__ __forceinline__ void finish_update(int op, const double (&tot)[2], do
 __global__ void __launch_bounds__(1024) k_bicg_finish(const double *__restrict__ part, int op, double *__restrict__ sc,
                                                       double tol, int maxit) {
     const double stopv = (op != FIN_INIT && op != FIN_DOT2) ? sc[SC_STOP] : 0.0;  // tested after the loads
+    if (op == FIN_BETA && stopv == STOP_HALF_PENDING) {  // the half-step update is done
+        if (threadIdx.x == 0) sc[SC_STOP] = STOP_HALF;
+        return;
+    }
     __shared__ double sh[2][16];
     double v[2] = {0, 0};
     const int ng = (op == FIN_ALPHA || op == FIN_SS) ? 1 : 2;
@@ -432,10 +436,10 @@ __global__ void __launch_bounds__(BS) k_b_bicg_xr(int op, const BCase *__restric
                                                   double *__restrict__ x, const double *__restrict__ Mp,
                                                   const double *__restrict__ Ms, double *__restrict__ r,
                                                   const double *__restrict__ s, const double *__restrict__ AMs,
-                                                  const double *__restrict__ rs, double *__restrict__ part, int k) {
+                                                  const double *__restrict__ rs, double *__restrict__ part) {
     const BSeg g = bseg(op, tab[m.cs[blockIdx.x]]);
     const double stopv = g.sc[SC_STOP];
-    const bool half = stopv == STOP_HALF && g.sc[SC_HALFK] == (double)k;
+    const bool half = stopv == STOP_HALF_PENDING;
     if (stopv != 0.0 && !half) return;
     const double alpha = g.sc[SC_ALPHA], omega = g.sc[SC_OMEGA];
     const int i = m.loc[blockIdx.x] * BS + threadIdx.x;
@@ -480,6 +484,10 @@ __global__ void __launch_bounds__(BS) k_b_finish(int op, const BCase *__restrict
                                                  int maxit) {
     const int c = blockIdx.x;
     double *sc = bseg(op, tab[c]).sc;
+    if (fin == FIN_BETA && sc[SC_STOP] == STOP_HALF_PENDING) {  // the half-step update is done
+        if (threadIdx.x == 0) sc[SC_STOP] = STOP_HALF;
+        return;
+    }
     if (fin != FIN_INIT && sc[SC_STOP] != 0.0) return;  // uniform per block
     const int ng = (fin == FIN_ALPHA || fin == FIN_SS) ? 1 : 2;
     const int b0 = first[c], b1 = first[c + 1];
@@ -507,9 +515,9 @@ void launch_b_bicg_s(int op, const BCase *tab, const BMapDev &m, const double *r
 }
 
 void launch_b_bicg_xr(int op, const BCase *tab, const BMapDev &m, double *x, const double *Mp, const double *Ms,
-                      double *r, const double *s_, const double *AMs, const double *rs, double *part, int k,
+                      double *r, const double *s_, const double *AMs, const double *rs, double *part,
                       hipStream_t s) {
-    hipLaunchKernelGGL(k_b_bicg_xr, dim3(m.nblk), dim3(BS), 0, s, op, tab, m, x, Mp, Ms, r, s_, AMs, rs, part, k);
+    hipLaunchKernelGGL(k_b_bicg_xr, dim3(m.nblk), dim3(BS), 0, s, op, tab, m, x, Mp, Ms, r, s_, AMs, rs, part);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -100,11 +100,11 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
         launch_bicg_s(n, r_, AMp_, s_, sc, part, s);
         launch_bicg_finish(part, FIN_SS, sc, s, tol, maxit);
     };
-    auto half2 = [&](int k) {
+    auto half2 = [&]() {
         M.apply(s_, Ms_, s);
         A.spmv_dot(Ms_, AMs_, s_, true, part, s);
         launch_bicg_finish(part, FIN_OMEGA, sc, s);
-        launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, k, s);
+        launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, s);
         launch_bicg_finish(part, FIN_BETA, sc, s, tol, maxit);
         launch_bicg_p(n, p_, r_, AMp_, sc, s);
     };
@@ -126,11 +126,13 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
             half1();
             if (exact_) {  // no launch past a half-step exit or breakdown
                 fetch();
-                if (host_sc_[SC_STOP] == STOP_HALF)  // the half-step update alone
-                    launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, it + j, s);
+                if (host_sc_[SC_STOP] == STOP_HALF_PENDING) {  // the half-step update alone
+                    launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, s);
+                    launch_bicg_finish(part, FIN_BETA, sc, s, tol, maxit);
+                }
                 if (host_sc_[SC_STOP] != 0.0) break;
             }
-            half2(it + j);
+            half2();
         }
         fetch();
         it = (int)host_sc_[SC_ITS];

@@ -16,7 +16,11 @@ constexpr int RED_GRID = 2048;  // blocks of every reducing kernel
 // it is set, so the host may queue several iterations before reading the block back.
 enum { SC_BB = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_BETA, SC_SS, SC_RR, SC_STOP, SC_ITS, SC_HALFK, SC_D0, SC_D1,
        SC_TOL, SC_COUNT = 16 };  // SC_TOL: the absolute tolerance of a batched solve
-enum { STOP_CONVERGED = 1, STOP_HALF = 2, STOP_BREAK_HALF = 3, STOP_BREAK_FULL = 4, STOP_MAXIT = 5 };
+// STOP_HALF_PENDING: the half step converged in this iteration and the x update of its
+// second half is still due (the iteration's FIN_BETA finish turns it into STOP_HALF), so
+// the launches of an iteration do not depend on its index
+enum { STOP_CONVERGED = 1, STOP_HALF = 2, STOP_BREAK_HALF = 3, STOP_BREAK_FULL = 4, STOP_MAXIT = 5,
+       STOP_HALF_PENDING = 6 };
 // finish-kernel operations
 enum { FIN_INIT = 0, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_BETA, FIN_DOT2 };
 
@@ -34,10 +38,10 @@ void launch_spmv_dot(const DCsr<double> &A, const double *x, double *y, const do
 void launch_bicg_init(const DCsr<double> &A, const double *x, const double *b, double *r, double *p, double *rs,
                       double *part, hipStream_t s);
 void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const double *sc, double *part, hipStream_t s);
-// x += alpha Mp + omega Ms, r = s - omega AMs with partials; in iteration k after a
-// half-step exit of that iteration, x += alpha Mp only
+// x += alpha Mp + omega Ms, r = s - omega AMs with partials; after a half-step exit of
+// the same iteration (STOP_HALF_PENDING), x += alpha Mp only
 void launch_bicg_xr(int n, double *x, const double *Mp, const double *Ms, double *r, const double *s_,
-                    const double *AMs, const double *rs, const double *sc, double *part, int k, hipStream_t s);
+                    const double *AMs, const double *rs, const double *sc, double *part, hipStream_t s);
 void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s);
 // sums the partials and updates the scalars; FIN_SS and FIN_BETA also apply the stopping
 // tests of the monitor (tol) and the iteration limit (maxit) and set SC_STOP

@@ -236,7 +236,10 @@ int cuibm_ensemble_run(int n, const char *const *case_folders, int argc, const c
  * stencil operators (the default), a DIAGONAL velocity and a SMOOTHED_AGGREGATION Poisson
  * preconditioner of one AMG precision, and equal tolerances, iteration limits and step
  * counts; otherwise returns CUIBM_ERR_UNSUPPORTED or CUIBM_ERR_ARGUMENT. `workers` host
- * threads (1..64) run the cases' set-up. Outputs as cuibm_ensemble_run. */
+ * threads (1..64) run the cases' set-up. argv may also hold -lockstepGraphs 0|1: with 1
+ * (the default) each solve's blocks of eight BiCGSTAB iterations run as a HIP graph
+ * captured at the block's first use and replayed after; with 0 their kernels are launched
+ * one by one. Outputs as cuibm_ensemble_run. */
 int cuibm_ensemble_lockstep(int n, const char *const *case_folders, int argc, const char *const *argv, int steps,
                             int workers, long long *cells, double *cd, double *cl, double *wall_s,
                             double *steps_wall_s, char *err, int err_len);

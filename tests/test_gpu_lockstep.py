@@ -7,7 +7,8 @@ At solver tolerance 1e-8 that can move a solve's stopping point by an iteration,
 and lift are compared within 1e-7 of the cases run alone with the same AMG precision over
 50 steps of the impulsive start, for the FP32 V-cycle and for the FP64 hierarchy; with
 solves to 1e-12 the FP64 batch's first case agrees with the oracle within 1e-8 after 20
-steps. Cases that the batch does not cover are refused with an error.
+steps. Graph replays of the solves' iteration blocks equal eager launches bit for bit. Cases
+that the batch does not cover are refused with an error.
 """
 import numpy as np
 import pytest
@@ -56,6 +57,20 @@ def test_lockstep_fp64_first_case_matches_oracle(tmp_path):
     print("case 0: lockstep C_D", r["cd"][0], "oracle", 2 * oi["forceX"], flush=True)
     assert abs(r["cd"][0] - 2 * oi["forceX"]) <= 1e-8 * max(1.0, abs(2 * oi["forceX"]))
     assert abs(r["cl"][0] - 2 * oi["forceY"]) <= 1e-8
+
+
+def test_lockstep_graphs_equal_eager_launches(tmp_path):
+    """the solves' eight-iteration blocks replayed as captured graphs give the results of
+    the same kernels launched one by one, bit for bit (every step replays each graph several
+    times, with the batch's own launches and read-backs between the replays)"""
+    steps = 30
+    paths, cases = ens.write_cases(str(tmp_path), n=5, seed=11, steps=steps, edges=(96, 220))
+    r1 = cuibm_amd.ensemble_run(paths, "-amgPrecision", "32", steps=steps, workers=4, lockstep=True)
+    r0 = cuibm_amd.ensemble_run(paths, "-amgPrecision", "32", "-lockstepGraphs", "0", steps=steps, workers=4,
+                                lockstep=True)
+    print(f"graphs {r1['steps_wall_s']:.3f} s, eager {r0['steps_wall_s']:.3f} s; C_D {np.round(r1['cd'], 6)}",
+          flush=True)
+    assert np.array_equal(r1["cd"], r0["cd"]) and np.array_equal(r1["cl"], r0["cl"])
 
 
 def test_lockstep_refuses_cases_it_does_not_cover(tmp_path):
