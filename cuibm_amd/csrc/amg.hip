@@ -406,7 +406,7 @@ struct AmgDev {
             // stencil finest level: x2 -> xt and r -> r, restriction, coarse cycle,
             // prolongation into xt, both post-sweeps straight into x
             AmgLevelDev<V, X> &L = lv[0], &C = lv[1];
-            if (fine.defl) launch_p5_deflate(fine, b, defl_part.p(), defl_c.p(), inv_wa, s);
+            if (fine.defl) launch_p5_wdot(fine, b, defl_part.p(), s);
             launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s, probe);
             launch_csr_op<V, X, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), X(0), s, stop);
             vcycle(1, C.b.p(), C.x.p(), s);
@@ -575,6 +575,8 @@ AmgPrecond::AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P
         f.defl = d.defl_c.p();
         f.defl_a0 = a0;
         f.defl_ac = ac;
+        f.defl_part = d.defl_part.p();
+        f.defl_inv_wa = d.inv_wa;
         d.probe = &im_->probe;
         d.prm = p;
         d.use_fine = true;
@@ -701,10 +703,8 @@ std::vector<double> AmgPrecond::launch_bytes() const {
         const double n = H[l].A.n, nnz = (double)H[l].A.ci.size();
         const double nc = H[l + 1].A.n, nnzP = (double)H[l].P.ci.size();
         if (l == 0 && fine_) {
-            if (fine_dev_.defl) {  // weighted sum of b, its one-block finish
+            if (fine_dev_.defl)  // partial weighted sums of b (the pre-smoothing kernel finishes them)
                 L.push_back(8.0 * n + 8.0 * (fine_dev_.nx + fine_dev_.ny));
-                L.push_back(8.0 * RED_GRID);
-            }
             L.push_back(level0_pre_bytes());
             L.push_back(spmv(nc, n, nnzP));                                          // restriction
             L.push_back(nnzP * (sv + 4) + 4 * (n + 1) + nc * sx + 2 * n * sx);        // prolong-add

@@ -321,6 +321,7 @@ LockstepBatch::LockstepBatch(const std::vector<NavierStokesSolver *> &cases, hip
         C.p5 = A[c]->fine_dev();
         C.p5.stop = C.scP + SC_STOP;
         C.p5.defl = deflate ? B.defl.p() + c : nullptr;
+        C.p5.defl_part = nullptr;  // the batch forms each case's c in k_b_wdot_finish
         if (deflate != (A[c]->fine_dev().defl != nullptr))
             throw Error("lockstep batch: the cases' AMG precisions must agree", ERR_ARGUMENT);
         C.fI = S.startI;

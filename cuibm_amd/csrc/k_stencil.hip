@@ -211,6 +211,30 @@ __device__ __forceinline__ void p5_pre_tile(const P5Dev &A, T w, const double *_
     const int I0 = (t % tiles_x) * TX, J0 = (t / tiles_x) * TY;
     load_coef(A, I0, J0, cf);
     const T *dinv = dinv_of<T>(A);
+    // the deflation constant c, wanted only where a pin lies in the tile or its halo
+    double cdefl = 0.0;
+    if (A.defl) {
+        auto near = [&](int ip) {
+            const int I = ip % A.nx, J = ip / A.nx;
+            return I >= I0 - 2 && I < I0 + TX + 2 && J >= J0 - 2 && J < J0 + TY + 2;
+        };
+        if (near(0) || (A.centre > 0 && near(A.centre))) {
+            if (A.defl_part) {  // uniform per block: the partials in a fixed order, as one finish block would
+                __shared__ double cs;
+                double acc[1] = {0};
+                for (int k = threadIdx.x; k < RED_GRID; k += 256) acc[0] += A.defl_part[k];
+                block_sum<1>(acc);
+                if (threadIdx.x == 0) {
+                    cs = acc[0] * A.defl_inv_wa;
+                    if (t == 0) A.defl[0] = cs;  // for the post-smoothing kernel
+                }
+                __syncthreads();
+                cdefl = cs;
+            } else {
+                cdefl = A.defl[0];
+            }
+        }
+    }
     T bv[R2], dv[R2];
 #pragma unroll
     for (int m = 0; m < R2; m++) {
@@ -219,7 +243,7 @@ __device__ __forceinline__ void p5_pre_tile(const P5Dev &A, T w, const double *_
         const bool in = k < N2 && inside(A, I, J);
         const int ip = in ? J * A.nx + I : 0;
         double bb = in ? b[ip] : 0.0;
-        if (A.defl && in && (ip == 0 || ip == A.centre)) bb -= A.defl[0] * (ip == 0 ? A.defl_a0 : A.defl_ac);
+        if (A.defl && in && (ip == 0 || ip == A.centre)) bb -= cdefl * (ip == 0 ? A.defl_a0 : A.defl_ac);
         bv[m] = (T)bb;
         dv[m] = in ? dinv[ip] : T(0);
     }
@@ -371,16 +395,6 @@ __global__ void __launch_bounds__(BS) k_p5_wdot(P5Dev A, const double *__restric
         acc[0] += (A.dx[ip % A.nx] * A.dy[ip / A.nx]) * r[ip];
     block_sum<1>(acc);
     if (threadIdx.x == 0) part[blk] = acc[0];
-}
-
-// one block: c = sum of the partials (fixed order) * inv_wa
-__global__ void __launch_bounds__(BS) k_p5_wdot_finish(const double *__restrict__ part, double *__restrict__ defl,
-                                                       double inv_wa, const double *__restrict__ stop) {
-    if (stop && *stop != 0.0) return;
-    double acc[1] = {0};
-    for (int k = threadIdx.x; k < RED_GRID; k += BS) acc[0] += part[k];
-    block_sum<1>(acc);
-    if (threadIdx.x == 0) defl[0] = acc[0] * inv_wa;
 }
 
 // one sweep xn = x + omega D^-1 (b - A x), a row per thread
@@ -629,11 +643,9 @@ void launch_p5_sweep2(const P5Dev &A, float omega, const float *x, const double 
     p5_sweep2<float>(A, omega, x, b, xo, s);
 }
 
-void launch_p5_deflate(const P5Dev &A, const double *r, double *part, double *defl, double inv_wa, hipStream_t s) {
+void launch_p5_wdot(const P5Dev &A, const double *r, double *part, hipStream_t s) {
     if (!A.dx || !A.dy) throw Error("deflation needs the cell widths of the stencil operator", ERR_ARGUMENT);
     hipLaunchKernelGGL(k_p5_wdot, dim3(RED_GRID), dim3(BS), 0, s, A, r, part);
-    CUIBM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_p5_wdot_finish, dim3(1), dim3(BS), 0, s, part, defl, inv_wa, A.stop);
     CUIBM_LAUNCH_CHECK();
 }
 

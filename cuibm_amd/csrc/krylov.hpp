@@ -225,9 +225,14 @@ struct P5Dev {
     const double *dx = nullptr, *dy = nullptr;  // cell widths (the cell areas weight the deflation)
     // FP64 deflation of the pinned constant mode for the FP32 V-cycle: defl[0] = c, the
     // input is r - c A 1 (A 1 is a0 at cell 0 and ac at the centre pin) and c is added to
-    // the output
-    const double *defl = nullptr;
+    // the output. With defl_part, the pre-smoothing kernel forms c itself from those
+    // RED_GRID partial sums of w_i r_i (c = their sum times defl_inv_wa) in the tiles whose
+    // halo holds a pin, and the tile of cell 0 stores it in defl[0] for the post-smoothing
+    // kernel; without, c is read from defl[0]
+    double *defl = nullptr;
     double defl_a0 = 0, defl_ac = 0;
+    const double *defl_part = nullptr;
+    double defl_inv_wa = 0;
 };
 
 class AmgPrecond : public Preconditioner {

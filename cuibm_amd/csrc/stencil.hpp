@@ -93,10 +93,10 @@ void launch_p5_sweep2(const P5Dev &A, double omega, const double *x, const doubl
 void launch_p5_pre(const P5Dev &A, float omega, const double *b, float *x2, float *r, hipStream_t s,
                    KernelProbe *probe = nullptr);
 void launch_p5_sweep2(const P5Dev &A, float omega, const float *x, const double *b, double *xo, hipStream_t s);
-// c = (sum_i w_i r_i) * inv_wa with w_i the area of cell i, into defl[0] (fixed-order
-// partial sums over RED_GRID blocks in part, then one block); returns at once while the
-// Krylov solve's stop code is set
-void launch_p5_deflate(const P5Dev &A, const double *r, double *part, double *defl, double inv_wa, hipStream_t s);
+// partial sums of w_i r_i (w = cell areas) over RED_GRID blocks, for the deflation
+// constant that the pre-smoothing kernel forms (P5Dev::defl_part); returns at once while
+// the Krylov solve's stop code is set
+void launch_p5_wdot(const P5Dev &A, const double *r, double *part, hipStream_t s);
 // one sweep xn = x + omega D^-1 (b - A x) (timing of the finest-level smoother)
 void launch_p5_sweep(const P5Dev &A, double omega, const double *x, const double *b, double *xn, hipStream_t s);
 
