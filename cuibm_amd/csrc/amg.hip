@@ -399,14 +399,15 @@ struct AmgDev {
             for (int j = 0; j < n; j++) inv[(size_t)i * n + j] -= sigma * v[i] * u[j] / uv;
     }
 
-    // the preconditioner: one V-cycle from x = 0 on the FP64 Krylov vectors b and x
-    void apply(const double *b, double *x, hipStream_t s) {
+    // the preconditioner: one V-cycle from x = 0 on the FP64 Krylov vectors b and x;
+    // weighted: the deflation's partial sums of b are already in defl_part
+    void apply(const double *b, double *x, hipStream_t s, bool weighted = false) {
         const X w = (X)prm.omega;
         if (use_fine && prm.npre == 2 && prm.npost == 2 && lv.size() > 1) {
             // stencil finest level: x2 -> xt and r -> r, restriction, coarse cycle,
             // prolongation into xt, both post-sweeps straight into x
             AmgLevelDev<V, X> &L = lv[0], &C = lv[1];
-            if (fine.defl) launch_p5_wdot(fine, b, defl_part.p(), s);
+            if (fine.defl && !weighted) launch_p5_wdot(fine, b, defl_part.p(), s);
             launch_p5_pre(fine, w, b, L.xt.p(), L.r.p(), s, probe);
             launch_csr_op<V, X, OP_SPMV>(L.lanesR, L.R, L.r.p(), nullptr, nullptr, C.b.p(), X(0), s, stop);
             vcycle(1, C.b.p(), C.x.p(), s);
@@ -613,6 +614,21 @@ void AmgPrecond::apply(const double *r, double *z, hipStream_t s) {
     else im_->d64.apply(r, z, s);
 }
 
+InputWeights AmgPrecond::input_weights() const {
+    InputWeights w;
+    if (!fine_dev_.defl) return w;
+    w.wx = fine_dev_.dx;
+    w.wy = fine_dev_.dy;
+    w.nx = fine_dev_.nx;
+    w.part = im_->d32.defl_part.p();
+    return w;
+}
+
+void AmgPrecond::apply_weighted(const double *r, double *z, hipStream_t s) {
+    if (fine_dev_.defl) im_->d32.apply(r, z, s, true);
+    else apply(r, z, s);
+}
+
 void AmgPrecond::level0_sweep(const double *b, const double *x, double *xn, hipStream_t s) {
     if (fine_) {
         launch_p5_sweep(fine_dev_, prm_.omega, x, b, xn, s);
@@ -703,8 +719,8 @@ std::vector<double> AmgPrecond::launch_bytes() const {
         const double n = H[l].A.n, nnz = (double)H[l].A.ci.size();
         const double nc = H[l + 1].A.n, nnzP = (double)H[l].P.ci.size();
         if (l == 0 && fine_) {
-            if (fine_dev_.defl)  // partial weighted sums of b (the pre-smoothing kernel finishes them)
-                L.push_back(8.0 * n + 8.0 * (fine_dev_.nx + fine_dev_.ny));
+            // (with deflation, the Krylov kernels that produce b also form its weighted partial
+            // sums, and the pre-smoothing kernel finishes them: no launch of its own)
             L.push_back(level0_pre_bytes());
             L.push_back(spmv(nc, n, nnzP));                                          // restriction
             L.push_back(nnzP * (sv + 4) + 4 * (n + 1) + nc * sx + 2 * n * sx);        // prolong-add

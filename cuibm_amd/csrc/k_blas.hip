@@ -178,11 +178,12 @@ void launch_bicg_init(const DCsr<double> &A, const double *x, const double *b, d
 }
 
 // s = r - alpha AMp with partials of (s.s)
+template <bool W>
 __global__ void __launch_bounds__(BS) k_bicg_s(int n, const double *__restrict__ r, const double *__restrict__ AMp,
                                                double *__restrict__ s, const double *__restrict__ sc,
-                                               double *__restrict__ part) {
+                                               double *__restrict__ part, InputWeights w) {
     const double stopv = sc[SC_STOP], alpha = sc[SC_ALPHA];
-    double acc[1] = {0};
+    double acc[2] = {0, 0};
     bool tested = false;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double si = r[i] - alpha * AMp[i];
@@ -192,15 +193,20 @@ __global__ void __launch_bounds__(BS) k_bicg_s(int n, const double *__restrict__
         }
         s[i] = si;
         acc[0] += si * si;
+        if (W) acc[1] += (w.wx[i % w.nx] * w.wy[i / w.nx]) * si;
     }
     if (stopv != 0.0) return;
-    block_sum<1>(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = acc[0];
+        if (W) w.part[blockIdx.x] = acc[1];
+    }
 }
 
 void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const double *sc, double *part,
-                   hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_s, dim3(RED_GRID), dim3(BS), 0, s, n, r, AMp, s_, sc, part);
+                   hipStream_t s, const InputWeights *w) {
+    if (w) hipLaunchKernelGGL(k_bicg_s<true>, dim3(RED_GRID), dim3(BS), 0, s, n, r, AMp, s_, sc, part, *w);
+    else hipLaunchKernelGGL(k_bicg_s<false>, dim3(RED_GRID), dim3(BS), 0, s, n, r, AMp, s_, sc, part, InputWeights{});
     CUIBM_LAUNCH_CHECK();
 }
 
@@ -258,8 +264,31 @@ __global__ void k_bicg_p(int n, double *__restrict__ p, const double *__restrict
     }
 }
 
-void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_p, dim3(grid_for(n)), dim3(BS), 0, s, n, p, r, AMp, sc);
+// the same with the weighted partial sums of the new p (a fixed grid of RED_GRID blocks)
+__global__ void __launch_bounds__(BS) k_bicg_pw(int n, double *__restrict__ p, const double *__restrict__ r,
+                                                const double *__restrict__ AMp, const double *__restrict__ sc,
+                                                InputWeights w) {
+    const double stopv = sc[SC_STOP], beta = sc[SC_BETA], omega = sc[SC_OMEGA];
+    double acc[1] = {0};
+    bool tested = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = r[i] + beta * p[i] - beta * omega * AMp[i];
+        if (!tested) {
+            if (stopv != 0.0) return;  // every thread returns here or below: no barrier is reached
+            tested = true;
+        }
+        p[i] = v;
+        acc[0] += (w.wx[i % w.nx] * w.wy[i / w.nx]) * v;
+    }
+    if (stopv != 0.0) return;
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) w.part[blockIdx.x] = acc[0];
+}
+
+void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s,
+                   const InputWeights *w) {
+    if (w) hipLaunchKernelGGL(k_bicg_pw, dim3(RED_GRID), dim3(BS), 0, s, n, p, r, AMp, sc, *w);
+    else hipLaunchKernelGGL(k_bicg_p, dim3(grid_for(n)), dim3(BS), 0, s, n, p, r, AMp, sc);
     CUIBM_LAUNCH_CHECK();
 }
 

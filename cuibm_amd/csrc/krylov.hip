@@ -92,21 +92,29 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
     const double rn0 = rn;
     int it = 0, restarts = 0;
     st.tolerance = tol;
-    // the first half of iteration k (through the test on ||s||), then the second half
+    // the first half of iteration k (through the test on ||s||), then the second half.
+    // A preconditioner that forms weighted sums of its input gets them from the kernels
+    // that produce s and p (for the p of a (re)start, it forms them itself).
+    const InputWeights iw = M.input_weights();
+    const InputWeights *wp = iw.part ? &iw : nullptr;
+    bool p_weighted = false;
     auto half1 = [&]() {
-        M.apply(p_, Mp_, s);
+        if (wp && p_weighted) M.apply_weighted(p_, Mp_, s);
+        else M.apply(p_, Mp_, s);
         A.spmv_dot(Mp_, AMp_, rs_, false, part, s);
         launch_bicg_finish(part, FIN_ALPHA, sc, s);
-        launch_bicg_s(n, r_, AMp_, s_, sc, part, s);
+        launch_bicg_s(n, r_, AMp_, s_, sc, part, s, wp);
         launch_bicg_finish(part, FIN_SS, sc, s, tol, maxit);
     };
     auto half2 = [&]() {
-        M.apply(s_, Ms_, s);
+        if (wp) M.apply_weighted(s_, Ms_, s);
+        else M.apply(s_, Ms_, s);
         A.spmv_dot(Ms_, AMs_, s_, true, part, s);
         launch_bicg_finish(part, FIN_OMEGA, sc, s);
         launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, s);
         launch_bicg_finish(part, FIN_BETA, sc, s, tol, maxit);
-        launch_bicg_p(n, p_, r_, AMp_, sc, s);
+        launch_bicg_p(n, p_, r_, AMp_, sc, s, wp);
+        p_weighted = wp != nullptr;
     };
     while (!(rn <= tol) && it < maxit) {
         // iterations to queue before the next read-back: a fraction of the estimate from
@@ -145,6 +153,7 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
         if (code == STOP_BREAK_HALF || code == STOP_BREAK_FULL) {
             if (++restarts > kMaxRestarts) break;
             start();  // x is untouched by a half-step breakdown
+            p_weighted = false;
             rn = std::sqrt(host_sc_[SC_RR]);
         }
     }

@@ -37,12 +37,23 @@ void launch_spmv_dot(const DCsr<double> &A, const double *x, double *y, const do
                      hipStream_t s, const double *stop = nullptr);
 void launch_bicg_init(const DCsr<double> &A, const double *x, const double *b, double *r, double *p, double *rs,
                       double *part, hipStream_t s);
-void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const double *sc, double *part, hipStream_t s);
+// weights w_i = wx[i % nx] * wy[i / nx] of a preconditioner's input (cell areas), whose
+// RED_GRID partial sums of w_i v_i the kernel producing the input v writes into part
+struct InputWeights {
+    const double *wx = nullptr, *wy = nullptr;
+    int nx = 0;
+    double *part = nullptr;
+};
+// s = r - alpha AMp with partials of (s.s); with w, also the weighted partials of s
+void launch_bicg_s(int n, const double *r, const double *AMp, double *s_, const double *sc, double *part, hipStream_t s,
+                   const InputWeights *w = nullptr);
 // x += alpha Mp + omega Ms, r = s - omega AMs with partials; after a half-step exit of
 // the same iteration (STOP_HALF_PENDING), x += alpha Mp only
 void launch_bicg_xr(int n, double *x, const double *Mp, const double *Ms, double *r, const double *s_,
                     const double *AMs, const double *rs, const double *sc, double *part, hipStream_t s);
-void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s);
+// p = r + beta (p - omega AMp); with w, also the weighted partials of p
+void launch_bicg_p(int n, double *p, const double *r, const double *AMp, const double *sc, hipStream_t s,
+                   const InputWeights *w = nullptr);
 // sums the partials and updates the scalars; FIN_SS and FIN_BETA also apply the stopping
 // tests of the monitor (tol) and the iteration limit (maxit) and set SC_STOP
 void launch_bicg_finish(const double *part, int op, double *sc, hipStream_t s, double tol = 0, int maxit = 0);
@@ -94,6 +105,10 @@ class Preconditioner {
   public:
     virtual ~Preconditioner() = default;
     virtual void apply(const double *r, double *z, hipStream_t s) = 0;
+    // the weights whose partial sums of each input the preconditioner forms (part null:
+    // none), and apply() for an input whose producer has already written them into part
+    virtual InputWeights input_weights() const { return {}; }
+    virtual void apply_weighted(const double *r, double *z, hipStream_t s) { apply(r, z, s); }
     virtual std::string name() const = 0;
     // bytes moved by one apply, counted as distinct arrays touched per kernel
     virtual double bytes_per_apply() const = 0;
@@ -243,6 +258,9 @@ class AmgPrecond : public Preconditioner {
     AmgPrecond(const HCsr &A, const AmgParams &p, hipStream_t s, const P5Dev *fine = nullptr);
     ~AmgPrecond() override;
     void apply(const double *r, double *z, hipStream_t s) override;
+    // the FP32 V-cycle's deflation weights (cell areas) on a stencil finest level
+    InputWeights input_weights() const override;
+    void apply_weighted(const double *r, double *z, hipStream_t s) override;
     void set_stop(const double *stop) override;
     std::string name() const override {
         return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)"
