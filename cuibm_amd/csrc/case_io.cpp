@@ -307,6 +307,14 @@ void readInputs(const std::string &caseFolder, int argc, const char *const *argv
             DB.PoissonSolve.amgCoarseMax = m;
             i++;
         }
+        else if (a == "-poissonRestart") {
+            char *end = nullptr;
+            const double f = std::strtod(v, &end);
+            if (end == v || *end != '\0' || !(f >= 0.0 && f < 1.0))
+                throw YamlError("-poissonRestart takes a factor in [0, 1)");
+            DB.PoissonSolve.restartReduction = f;
+            i++;
+        }
         else if (a == "-amgRefresh") {
             const std::string o = v;
             if (o != "0" && o != "1") throw YamlError("-amgRefresh takes 0 or 1, not '" + o + "'");

@@ -36,6 +36,10 @@ struct LinearSolverParams {
     int amgPrecision = 64;
     // largest coarsest AMG level (a dense inverse); 0: the solver's default
     int amgCoarseMax = 0;
+    // BiCGSTAB restarts from the true residual each time its residual has fallen by this
+    // factor (reliable updates); < 0: the preconditioner's default (1e-3 for the FP32
+    // V-cycle, never otherwise), 0: never
+    double restartReduction = -1.0;
     // a moving body's hierarchy whose matrix kept its pattern: set up again numerically on
     // the device (true) or built anew on the host (false)
     bool amgRefresh = true;

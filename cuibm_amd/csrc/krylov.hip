@@ -116,46 +116,70 @@ SolveStat Bicgstab::solve(const LinOp &A, double *x, const double *b, Preconditi
         launch_bicg_p(n, p_, r_, AMp_, sc, s, wp);
         p_weighted = wp != nullptr;
     };
-    while (!(rn <= tol) && it < maxit) {
-        // iterations to queue before the next read-back: a fraction of the estimate from
-        // the residual reduction so far (or of the last solve), so that few launches
-        // fall past the end; one in exact mode
-        int batch = 1;
-        if (!exact_) {
-            double rate = rate_;
-            if (it > 0 && rn < rn0) rate = std::pow(rn / rn0, 1.0 / it);
-            if (rate > 0 && rate < 1) {
-                const double left = std::log(tol / rn) / std::log(rate);
-                batch = (int)std::max(1.0, std::min(16.0, std::floor(0.7 * left)));
-            }
-            batch = std::min(batch, maxit - it);
-        }
-        for (int j = 0; j < batch; j++) {
-            half1();
-            if (exact_) {  // no launch past a half-step exit or breakdown
-                fetch();
-                if (host_sc_[SC_STOP] == STOP_HALF_PENDING) {  // the half-step update alone
-                    launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, s);
-                    launch_bicg_finish(part, FIN_BETA, sc, s, tol, maxit);
+    // The recurrence residual r can drift from b - A x through rounding when a solve runs
+    // many iterations with large updates (seen with the FP32 V-cycle at tolerances near
+    // 1e-12: after thousands of iterations r claimed 5e-11 of ||b|| while b - A x was
+    // larger than b). A solve that converged after kVerifyIts iterations or a restart
+    // therefore forms b - A x once; more than kDriftFactor times the tolerance restarts
+    // the iteration from it (counted as a restart), at most kMaxChecks times.
+    int checks = 0;
+    const double reliable = M.restart_reduction();
+    double rn_ref = rn;  // the true residual at the last (re)start
+    for (;;) {
+        while (!(rn <= tol) && it < maxit) {
+            // iterations to queue before the next read-back: a fraction of the estimate from
+            // the residual reduction so far (or of the last solve), so that few launches
+            // fall past the end; one in exact mode
+            int batch = 1;
+            if (!exact_) {
+                double rate = rate_;
+                if (it > 0 && rn < rn0) rate = std::pow(rn / rn0, 1.0 / it);
+                if (rate > 0 && rate < 1) {
+                    const double left = std::log(tol / rn) / std::log(rate);
+                    batch = (int)std::max(1.0, std::min(16.0, std::floor(0.7 * left)));
                 }
-                if (host_sc_[SC_STOP] != 0.0) break;
+                batch = std::min(batch, maxit - it);
             }
-            half2();
-        }
-        fetch();
-        it = (int)host_sc_[SC_ITS];
-        const int code = (int)host_sc_[SC_STOP];
-        if (code == STOP_HALF) {
-            rn = std::sqrt(host_sc_[SC_SS]);
-            break;
-        }
-        rn = std::sqrt(host_sc_[SC_RR]);
-        if (code == STOP_BREAK_HALF || code == STOP_BREAK_FULL) {
-            if (++restarts > kMaxRestarts) break;
-            start();  // x is untouched by a half-step breakdown
-            p_weighted = false;
+            for (int j = 0; j < batch; j++) {
+                half1();
+                if (exact_) {  // no launch past a half-step exit or breakdown
+                    fetch();
+                    if (host_sc_[SC_STOP] == STOP_HALF_PENDING) {  // the half-step update alone
+                        launch_bicg_xr(n, x, Mp_, Ms_, r_, s_, AMs_, rs_, sc, part, s);
+                        launch_bicg_finish(part, FIN_BETA, sc, s, tol, maxit);
+                    }
+                    if (host_sc_[SC_STOP] != 0.0) break;
+                }
+                half2();
+            }
+            fetch();
+            it = (int)host_sc_[SC_ITS];
+            const int code = (int)host_sc_[SC_STOP];
+            if (code == STOP_HALF) {
+                rn = std::sqrt(host_sc_[SC_SS]);
+                break;
+            }
             rn = std::sqrt(host_sc_[SC_RR]);
+            if (code == STOP_BREAK_HALF || code == STOP_BREAK_FULL) {
+                if (++restarts > kMaxRestarts) break;
+                start();  // x is untouched by a half-step breakdown
+                p_weighted = false;
+                rn = rn_ref = std::sqrt(host_sc_[SC_RR]);
+            } else if (reliable > 0 && code == 0 && rn < reliable * rn_ref) {
+                if (++restarts > kMaxRestarts) break;
+                start();  // reliable update: the true residual, new search directions
+                p_weighted = false;
+                rn = rn_ref = std::sqrt(host_sc_[SC_RR]);
+            }
         }
+        if (!(rn <= tol) || (it < kVerifyIts && restarts == 0) || checks >= kMaxChecks) break;
+        checks++;
+        start();
+        p_weighted = false;
+        const double tr = std::sqrt(host_sc_[SC_RR]);
+        if (tr <= kDriftFactor * tol) break;
+        rn = rn_ref = tr;
+        if (++restarts > kMaxRestarts) break;
     }
     if (it > 0 && rn > 0 && rn0 > 0 && rn < rn0) rate_ = std::pow(rn / rn0, 1.0 / it);
     A.set_stop(nullptr);

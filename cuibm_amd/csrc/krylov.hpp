@@ -109,6 +109,11 @@ class Preconditioner {
     // none), and apply() for an input whose producer has already written them into part
     virtual InputWeights input_weights() const { return {}; }
     virtual void apply_weighted(const double *r, double *z, hipStream_t s) { apply(r, z, s); }
+    // a preconditioner whose rounding is far coarser than the solve's (the FP32 V-cycle)
+    // is not the same linear map from one apply to the next; BiCGSTAB then restarts from
+    // the true residual b - A x whenever its residual has fallen by this factor since the
+    // last (re)start (reliable updates); 0: never
+    virtual double restart_reduction() const { return 0.0; }
     virtual std::string name() const = 0;
     // bytes moved by one apply, counted as distinct arrays touched per kernel
     virtual double bytes_per_apply() const = 0;
@@ -172,6 +177,10 @@ struct AmgParams {
     // body rows moved keep to its pattern (0: the pattern of A itself)
     int pad_nx = 0;
     int pad_margin = 8;
+    // reliable-update factor of the BiCGSTAB it preconditions (restart_reduction());
+    // < 0: 1e-3 for the FP32 V-cycle (the best of 1e-1 .. 1e-6 on the 1024^2 cylinder), 0
+    // otherwise
+    double restart = -1.0;
 };
 
 // HIP events around the launches of one kernel inside the live solve: mark() before
@@ -261,6 +270,9 @@ class AmgPrecond : public Preconditioner {
     // the FP32 V-cycle's deflation weights (cell areas) on a stencil finest level
     InputWeights input_weights() const override;
     void apply_weighted(const double *r, double *z, hipStream_t s) override;
+    double restart_reduction() const override {
+        return prm_.restart >= 0 ? prm_.restart : prm_.precision == 32 ? 1e-3 : 0.0;
+    }
     void set_stop(const double *stop) override;
     std::string name() const override {
         return prm_.precision == 32 ? "SMOOTHED_AGGREGATION(FP32)"
@@ -360,6 +372,8 @@ class Bicgstab {
 
   private:
     static constexpr int kMaxRestarts = 64;
+    static constexpr int kVerifyIts = 100, kMaxChecks = 3;
+    static constexpr double kDriftFactor = 1e3;
     // vector stride in the external workspace: a multiple of 32 doubles (256 bytes)
     static size_t stride(int n) { return ((size_t)n + 31) / 32 * 32; }
     int n_;

@@ -167,6 +167,7 @@ static std::unique_ptr<Preconditioner> make_precond(const LinearSolverParams &p,
             ap.fused_restriction = fused_restriction;
             ap.refreshable = !fused_restriction;  // rebuilt every step: keep the symbolic structure
             ap.pad_nx = ap.refreshable ? pad_nx : 0;
+            ap.restart = p.restartReduction;
             return std::unique_ptr<Preconditioner>(new AmgPrecond(download_csr(A, s), ap, s, fine));
         }
         default: throw Error("preconditioner AINV is not supported", ERR_UNSUPPORTED);

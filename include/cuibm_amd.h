@@ -53,7 +53,10 @@ const char *cuibm_last_error(void);
  * -amgRefresh 0|1 (a moving embedded body's Poisson hierarchy is built on a pattern padded
  * around the body's rows and, while each new matrix keeps to it, set up again numerically
  * on the device with its aggregates kept: 1, the default; or built anew on the host every
- * step, as the reference does: 0), -poissonPreconditioner NAME,
+ * step, as the reference does: 0), -poissonRestart F (the Poisson BiCGSTAB restarts from its
+ * true residual each time its residual has fallen by F since the last restart, 0 <= F < 1,
+ * 0: never; default 1e-3 with the FP32 V-cycle, whose rounding makes it a slightly different
+ * map at every apply, and never otherwise), -poissonPreconditioner NAME,
  * -operators csr|stencil, -outputFolder DIR and -output 0|1) and construct the solver
  * (not yet initialised). */
 int cuibm_create(const char *case_folder, int argc, const char *const *argv, cuibm_solver **out);

@@ -124,6 +124,15 @@ def test_amg_coarse_max_is_parsed_and_bounded():
         assert e.value.code == -2 and "16..6144" in str(e.value)
 
 
+def test_poisson_restart_is_parsed_and_bounded():
+    for ok in ("0", "1e-5", "0.5"):
+        cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-poissonRestart", ok).close()
+    for bad in ("-1", "1", "x"):
+        with pytest.raises(cuibm_amd.CuibmError) as e:
+            cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-poissonRestart", bad)
+        assert e.value.code == -2 and "[0, 1)" in str(e.value)
+
+
 def test_amg_refresh_switch_is_parsed():
     for ok in ("0", "1"):
         cuibm_amd.Solver(os.path.join(CASES, "cylinderRe40_small_luo"), "-amgRefresh", ok).close()

@@ -14,7 +14,7 @@ Bars: u within 1e-10 relative L2 per step; p within 1e-10 with its mean removed 
 mean within 1e-8 (the Poisson operator is pinned by one doubled diagonal, so its constant
 mode is fixed only weakly; rounding moves it by up to 1e-9 while the velocity, which sees
 pressure differences only, agrees to 1e-12); drag and lift within 1e-8 in FP64; the mixed
-form within |dC_D| <= 1e-6 of the FP64 oracle.
+form within |dC_D| <= 1e-6 of the FP64 oracle, its p (mean removed) within 1e-9.
 """
 import json
 import os
@@ -33,20 +33,23 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 TIGHT = {"velocityTol": 1e-12, "poissonTol": 1e-12}
 
 
-def _check_fields(g, o, k):
+def _check_fields(g, o, k, p_bar=1e-10):
     du = rel_l2(g.get("u"), o.get("u"))
     pg, po_ = g.get("p"), o.get("p")
     dp = rel_l2(pg - pg.mean(), po_ - po_.mean())
     dm = abs(pg.mean() - po_.mean()) / np.sqrt(np.mean(po_ ** 2))
     assert du < 1e-10, (k, "u", du)
-    assert dp < 1e-10, (k, "p", dp)
+    assert dp < p_bar, (k, "p", dp)
     assert dm < 1e-8, (k, "p mean", dm)
     return du, dp
 
 
 def test_cylinder_1024_mixed_and_fp64_match_oracle():
     """The 1024x1024 Re = 550 cylinder over 10 steps of its impulsive start: the FP64 and
-    the FP32 V-cycle forms both against the FP64 oracle after every step."""
+    the FP32 V-cycle forms both against the FP64 oracle after every step. The mixed form's
+    Poisson solves restart from the true residual as they go (reliable updates), so they
+    stop at other points of the 1e-12 ball than the oracle's: its p (mean removed) is held
+    to 1e-9, and its drag and lift to 1e-6; its u keeps the 1e-10 bar."""
     name = "cylinderRe550_1024_luo"
     args = ["-velocityTol", "1e-12", "-poissonTol", "1e-12"]
     g64 = cuibm_amd.Solver(case(name), *args, "-amgPrecision", "64", diagnostics=False)
@@ -60,7 +63,7 @@ def test_cylinder_1024_mixed_and_fp64_match_oracle():
         o.step()
         i64, i32, io = g64.info(), g32.info(), o.info()
         d64 = _check_fields(g64, o, k)
-        d32 = _check_fields(g32, o, k)
+        d32 = _check_fields(g32, o, k, p_bar=1e-9)
         cd = 2 * io["forceX"]
         print(f"step {k + 1}: C_D oracle {cd:.12f} fp64 {2 * i64['forceX']:.12f} mixed {2 * i32['forceX']:.12f}"
               f" | u {d64[0]:.1e} {d32[0]:.1e} p {d64[1]:.1e} {d32[1]:.1e} | Poisson its {i64['iter2']:.0f}"
