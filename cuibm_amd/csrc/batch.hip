@@ -125,6 +125,11 @@ struct LockstepBatch::Impl {
     static constexpr int GRAPH_ITS = 8;
     bool graphs = true;
     hipGraphExec_t block[2] = {nullptr, nullptr};
+    // reliable updates of the Poisson solve (the cases' restart_reduction()): a case whose
+    // residual has fallen by this factor since its last (re)start restarts from b - A x
+    double reliable2 = 0.0;  // the factor squared (compared with r.r)
+    DArray<int> mask;
+    std::vector<int> host_mask;
 
     ~Impl() {
         for (hipGraphExec_t &g : block)
@@ -164,12 +169,37 @@ struct LockstepBatch::Impl {
             launch_b_bicg_p(op, tab.p(), md, w.p.p(), w.r.p(), w.AMp.p(), s);
         };
         int it = 0;
+        const double rel2 = op == BOP_POIS ? reliable2 : 0.0;
+        std::vector<double> rr_ref;
         for (;;) {
             fetch();
+            if (rr_ref.empty())
+                for (int c = 0; c < nc; c++) rr_ref.push_back(scv(c, op)[SC_RR]);
             bool running = false;
             for (int c = 0; c < nc; c++)
                 if (scv(c, op)[SC_STOP] == 0.0) running = true;
             if (!running || it >= maxit) break;
+            if (rel2 > 0) {  // reliable updates of the running cases whose residual fell far enough
+                bool any = false;
+                for (int c = 0; c < nc; c++) {
+                    const double *v = scv(c, op);
+                    host_mask[c] = v[SC_STOP] == 0.0 && v[SC_RR] < rel2 * rr_ref[c];
+                    any = any || host_mask[c];
+                }
+                if (any) {
+                    mask.upload(host_mask, s);
+                    launch_b_init(op, tab.p(), md, x, b, w.r.p(), w.p.p(), w.rs.p(), w.part.p(), s, mask.p());
+                    launch_b_finish(op, tab.p(), nc, m.first(), md.nblk, w.part.p(), FIN_RESTART, rtol, maxit, s,
+                                    mask.p());
+                    fetch();
+                    for (int c = 0; c < nc; c++)
+                        if (host_mask[c]) rr_ref[c] = scv(c, op)[SC_RR];
+                    running = false;
+                    for (int c = 0; c < nc; c++)
+                        if (scv(c, op)[SC_STOP] == 0.0) running = true;
+                    if (!running) break;
+                }
+            }
             const int batch = std::min(GRAPH_ITS, maxit - it);
             if (graphs && batch == GRAPH_ITS) {
                 // the block reads and writes the same arrays every time (x and the
@@ -379,6 +409,12 @@ LockstepBatch::LockstepBatch(const std::vector<NavierStokesSolver *> &cases, hip
     B.fout.alloc((size_t)6 * B.nc);
     launch_fill(B.fout.p(), 0.0, 6 * B.nc, s);
     B.amg.reset(new BatchAmg(A, B.offP, B.tab.p(), B.tab_host, B.mP.dev(), B.mP.first(), s));
+    const double f = A[0]->restart_reduction();
+    for (auto *a : A)
+        if (a->restart_reduction() != f) throw Error("lockstep batch: cases of different -poissonRestart", ERR_ARGUMENT);
+    B.reliable2 = f * f;
+    B.mask.alloc(B.nc);
+    B.host_mask.assign(B.nc, 0);
     CUIBM_HIP(hipStreamSynchronize(s));
 }
 

@@ -73,8 +73,9 @@ void launch_b_project(const BCase *tab, const BMapDev &m, double *u, const doubl
 // which operator of the case table: the velocity (V5) or the Poisson (P5) stencil
 enum { BOP_VEL = 0, BOP_POIS = 1 };
 // r = b - A x, p = rs = r, partials of (r.r, b.b) per block
+// (with mask: only the cases c with mask[c] != 0)
 void launch_b_init(int op, const BCase *tab, const BMapDev &m, const double *x, const double *b, double *r, double *p,
-                   double *rs, double *part, hipStream_t s);
+                   double *rs, double *part, hipStream_t s, const int *mask = nullptr);
 // y = A x with partials of (y.a) [and (y.y)] per block; returns at once for stopped cases
 void launch_b_spmv_dot(int op, const BCase *tab, const BMapDev &m, const double *x, double *y, const double *a,
                        bool want_yy, double *part, hipStream_t s);
@@ -88,7 +89,7 @@ void launch_b_bicg_p(int op, const BCase *tab, const BMapDev &m, double *p, cons
 // per case: sum its blocks' partials and update its scalars (one block per case); FIN_INIT
 // also sets the case's absolute tolerance rtol * ||b|| in SC_TOL
 void launch_b_finish(int op, const BCase *tab, int ncases, const int *first, int nblk, const double *part, int fin,
-                     double rtol, int maxit, hipStream_t s);
+                     double rtol, int maxit, hipStream_t s, const int *mask = nullptr);
 // z = dinv r over the concatenated velocity arrays (stopped cases included)
 void launch_b_jacobi(long n, const double *dinv, const double *r, double *z, hipStream_t s);
 

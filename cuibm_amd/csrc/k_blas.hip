@@ -510,15 +510,16 @@ __global__ void __launch_bounds__(BS) k_b_bicg_p(int op, const BCase *__restrict
 // one block per case: the case's block partials in a fixed order, then finish_update
 __global__ void __launch_bounds__(BS) k_b_finish(int op, const BCase *__restrict__ tab, const int *__restrict__ first,
                                                  int nblk, const double *__restrict__ part, int fin, double rtol,
-                                                 int maxit) {
+                                                 int maxit, const int *__restrict__ mask) {
     const int c = blockIdx.x;
+    if (mask && !mask[c]) return;  // uniform per block
     double *sc = bseg(op, tab[c]).sc;
     if (fin == FIN_BETA && sc[SC_STOP] == STOP_HALF_PENDING) {  // the half-step update is done
         if (threadIdx.x == 0) sc[SC_STOP] = STOP_HALF;
         return;
     }
     if (fin != FIN_INIT && sc[SC_STOP] != 0.0) return;  // uniform per block
-    const int ng = (fin == FIN_ALPHA || fin == FIN_SS) ? 1 : 2;
+    const int ng = (fin == FIN_ALPHA || fin == FIN_SS) ? 1 : 2;  // FIN_RESTART: (r.r, b.b)
     const int b0 = first[c], b1 = first[c + 1];
     double v[2] = {0, 0};
     for (int g = 0; g < ng; g++)
@@ -532,6 +533,10 @@ __global__ void __launch_bounds__(BS) k_b_finish(int op, const BCase *__restrict
         finish_update(fin, tot, sc, 0.0, maxit);
         sc[SC_TOL] = rtol * sqrt(sc[SC_BB]);
         if (sqrt(sc[SC_RR]) <= sc[SC_TOL]) sc[SC_STOP] = STOP_CONVERGED;  // converged at the start
+    } else if (fin == FIN_RESTART) {  // new recurrences from the true residual
+        sc[SC_RR] = tot[0];
+        sc[SC_RHO] = tot[0];
+        if (sqrt(tot[0]) <= sc[SC_TOL]) sc[SC_STOP] = STOP_CONVERGED;
     } else {
         finish_update(fin, tot, sc, sc[SC_TOL], maxit);
     }
@@ -557,8 +562,8 @@ void launch_b_bicg_p(int op, const BCase *tab, const BMapDev &m, double *p, cons
 }
 
 void launch_b_finish(int op, const BCase *tab, int ncases, const int *first, int nblk, const double *part, int fin,
-                     double rtol, int maxit, hipStream_t s) {
-    hipLaunchKernelGGL(k_b_finish, dim3(ncases), dim3(BS), 0, s, op, tab, first, nblk, part, fin, rtol, maxit);
+                     double rtol, int maxit, hipStream_t s, const int *mask) {
+    hipLaunchKernelGGL(k_b_finish, dim3(ncases), dim3(BS), 0, s, op, tab, first, nblk, part, fin, rtol, maxit, mask);
     CUIBM_LAUNCH_CHECK();
 }
 

@@ -671,7 +671,8 @@ template <int OP>
 __global__ void __launch_bounds__(BS) k_b_init(const BCase *__restrict__ tab, BMapDev m, const double *__restrict__ x,
                                                const double *__restrict__ b, double *__restrict__ r,
                                                double *__restrict__ p, double *__restrict__ rs,
-                                               double *__restrict__ part) {
+                                               double *__restrict__ part, const int *__restrict__ mask) {
+    if (mask && !mask[m.cs[blockIdx.x]]) return;  // uniform per block
     const BCase &C = tab[m.cs[blockIdx.x]];
     const long off = OP == BOP_VEL ? C.offUV : C.offP;
     const int n = OP == BOP_VEL ? C.numUV : C.numP;
@@ -719,11 +720,11 @@ __global__ void __launch_bounds__(BS) k_b_spmv_dot(const BCase *__restrict__ tab
 }
 
 void launch_b_init(int op, const BCase *tab, const BMapDev &m, const double *x, const double *b, double *r, double *p,
-                   double *rs, double *part, hipStream_t s) {
+                   double *rs, double *part, hipStream_t s, const int *mask) {
     if (op == BOP_VEL)
-        hipLaunchKernelGGL(k_b_init<BOP_VEL>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, b, r, p, rs, part);
+        hipLaunchKernelGGL(k_b_init<BOP_VEL>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, b, r, p, rs, part, mask);
     else
-        hipLaunchKernelGGL(k_b_init<BOP_POIS>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, b, r, p, rs, part);
+        hipLaunchKernelGGL(k_b_init<BOP_POIS>, dim3(m.nblk), dim3(BS), 0, s, tab, m, x, b, r, p, rs, part, mask);
     CUIBM_LAUNCH_CHECK();
 }
 

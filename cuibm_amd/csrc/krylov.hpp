@@ -22,7 +22,9 @@ enum { SC_BB = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_BETA, SC_SS, SC_RR, SC_STOP, SC
 enum { STOP_CONVERGED = 1, STOP_HALF = 2, STOP_BREAK_HALF = 3, STOP_BREAK_FULL = 4, STOP_MAXIT = 5,
        STOP_HALF_PENDING = 6 };
 // finish-kernel operations
-enum { FIN_INIT = 0, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_BETA, FIN_DOT2 };
+// FIN_RESTART (batched solves): a restart of the cases marked in a mask, from the true
+// residual; the iteration count, the tolerance and the half-step record are kept
+enum { FIN_INIT = 0, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_BETA, FIN_DOT2, FIN_RESTART };
 
 // ---- k_blas.hip
 void launch_fill(double *x, double v, long n, hipStream_t s);
