@@ -208,7 +208,14 @@ struct LockstepBatch::Impl {
                 if (!g) {
                     hipGraph_t graph = nullptr;
                     CUIBM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    for (int j = 0; j < GRAPH_ITS; j++) iteration();
+                    try {
+                        for (int j = 0; j < GRAPH_ITS; j++) iteration();
+                    } catch (...) {  // leave the stream out of capture mode
+                        hipGraph_t partial = nullptr;
+                        (void)hipStreamEndCapture(s, &partial);
+                        if (partial) (void)hipGraphDestroy(partial);
+                        throw;
+                    }
                     CUIBM_HIP(hipStreamEndCapture(s, &graph));
                     const hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(graph);
