@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
-    for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+    for (int i = xcd_block(gridDim.x) * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restric
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
-    for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+    for (int i = xcd_block(gridDim.x) * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
@@ -112,7 +112,7 @@ void launch_rres(int lpr, const DCsr<V> &G, int nf, const X *x, const X *b, X *y
     const int n = G.n;
     if (n <= 0) return;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
-    const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
+    const dim3 g(xcd_blocks(blocks)), t(BS);
     switch (lpr) {
         case 1: hipLaunchKernelGGL((k_rres<V, X, 1>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop, seg, cstop); break;
         case 4: hipLaunchKernelGGL((k_rres<V, X, 4>), g, t, 0, s, n, nf, G.rp.p(), G.ci.p(), G.v.p(), x, b, y, stop, seg, cstop); break;
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, 
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
-    for (int i = blockIdx.x * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+    for (int i = xcd_block(gridDim.x) * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
@@ -160,7 +160,7 @@ void launch_pre2(int lpr, const DCsr<V> &A, const V *dinv, const X *b, X *x2, X 
                  const double *stop, const int *seg = nullptr, const double *const *cstop = nullptr) {
     const int n = A.n;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
-    const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
+    const dim3 g(xcd_blocks(blocks)), t(BS);
     switch (lpr) {
         case 1: hipLaunchKernelGGL((k_pre2<V, X, 1>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop, seg, cstop); break;
         case 4: hipLaunchKernelGGL((k_pre2<V, X, 4>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), dinv, b, x2, omega, stop, seg, cstop); break;
@@ -187,7 +187,7 @@ void launch_csr_op(int lpr, const DCsr<V> &A, const X *x, const X *b, const V *d
     const int n = A.n;
     if (n <= 0) return;
     const long blocks = ((long)n * lpr + BS - 1) / BS;
-    const dim3 g((unsigned)std::min<long>(blocks, 65536)), t(BS);
+    const dim3 g(xcd_blocks(blocks)), t(BS);
     switch (lpr) {
         case 1: hipLaunchKernelGGL((k_csr<V, X, 1, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop, seg, cstop); break;
         case 4: hipLaunchKernelGGL((k_csr<V, X, 4, OP>), g, t, 0, s, n, A.rp.p(), A.ci.p(), A.v.p(), x, b, dinv, y, omega, stop, seg, cstop); break;

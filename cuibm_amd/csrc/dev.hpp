@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -37,6 +38,17 @@ enum ErrorCode {
 
 constexpr int BS = 256;
 inline int nblk(long n, int bs = BS) { return (int)((n + bs - 1) / bs); }
+
+// The MI355X hands workgroups to its 8 XCDs (each with its own L2) round-robin. With a
+// grid that is a multiple of 8, block b -> (b % 8) * (grid / 8) + b / 8 is a permutation
+// that gives each XCD a contiguous range of the launch's work, so rows that gather
+// from neighbouring indices find them in their own XCD's L2.
+__device__ __forceinline__ int xcd_block(int grid) { return (int)(blockIdx.x % 8) * (grid / 8) + (int)(blockIdx.x / 8); }
+// grid of at least `blocks` workgroups, a multiple of 8, at most 65536
+inline unsigned xcd_blocks(long blocks) {
+    const long b = std::min<long>(std::max<long>(blocks, 1), 65536);
+    return (unsigned)((b + 7) / 8 * 8);
+}
 
 template <class T>
 class DArray {
