@@ -61,7 +61,10 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
-    for (int i = xcd_block(gridDim.x) * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+    // rows banded by XCD, except in a batch, where the cases that stop early would leave
+    // whole XCDs idle (blocks in dispatch order spread each case over all of them)
+    const int blk = seg ? (int)blockIdx.x : xcd_block(gridDim.x);
+    for (int i = blk * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
@@ -90,7 +93,10 @@ __global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restric
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
-    for (int i = xcd_block(gridDim.x) * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+    // rows banded by XCD, except in a batch, where the cases that stop early would leave
+    // whole XCDs idle (blocks in dispatch order spread each case over all of them)
+    const int blk = seg ? (int)blockIdx.x : xcd_block(gridDim.x);
+    for (int i = blk * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
@@ -136,7 +142,10 @@ __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, 
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row's loads
     const int lane = threadIdx.x & (LPR - 1);
     const int per_block = BS / LPR;
-    for (int i = xcd_block(gridDim.x) * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
+    // rows banded by XCD, except in a batch, where the cases that stop early would leave
+    // whole XCDs idle (blocks in dispatch order spread each case over all of them)
+    const int blk = seg ? (int)blockIdx.x : xcd_block(gridDim.x);
+    for (int i = blk * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
