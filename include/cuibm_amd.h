@@ -234,8 +234,9 @@ int cuibm_ensemble_run(int n, const char *const *case_folders, int argc, const c
 
 /* No counterpart in the reference: the same run in lockstep. Every phase of the time step
  * is one launch over all n cases (fields in concatenated arrays, a block map giving each
- * block its case); the BiCGSTAB solves keep per-case scalars and stopping tests, so each
- * case stops at its own tolerance. The cases must be LUO with static bodies at rest,
+ * block its case); the BiCGSTAB solves keep per-case scalars, stopping tests and (for the
+ * Poisson solve, as -poissonRestart sets them) reliable-update restarts, so each case stops
+ * at its own tolerance. The cases must be LUO with static bodies at rest,
  * stencil operators (the default), a DIAGONAL velocity and a SMOOTHED_AGGREGATION Poisson
  * preconditioner of one AMG precision, and equal tolerances, iteration limits and step
  * counts; otherwise returns CUIBM_ERR_UNSUPPORTED or CUIBM_ERR_ARGUMENT. `workers` host
