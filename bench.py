@@ -400,6 +400,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="initialise the case, run two time steps and the copy calibration kernel "
                          "(for hardware-counter runs)")
+    ap.add_argument("--kernels-case", default="",
+                    help="with --kernels-only: this case folder instead, and its operator products timed")
     ap.add_argument("--published", action="store_true",
                     help="run the reference's published whole-run workloads in full and print their times "
                          "beside the published ones (one JSON line; not the headline metric)")
@@ -445,13 +447,15 @@ def main():
                           "runs": rows}), flush=True)
         return
     if args.kernels_only:
-        # two live time steps (their kernels are the ones the counters should see), then
-        # the copy calibration kernel
-        s = cu.Solver(case, diagnostics=False)
+        # two live time steps (their kernels are the ones the counters should see), then,
+        # for another case, the two operator products with their dots as launched in the
+        # solves (1 + reps launches each, in this order), then the copy calibration kernel
+        other = bool(args.kernels_case)
+        s = cu.Solver(os.path.join(cu.CASES, args.kernels_case) if other else case, diagnostics=False)
         s.initialise()
         s.step_time(2)
         out = {}
-        for k in ("copy",):
+        for k in (("spmv_dot_lhs1", "spmv_dot_lhs2") if other else ()) + ("copy",):
             ms, b = s.time_kernel(k, args.kernel_reps)
             out[k] = {"us": ms * 1e3, "bytes": b, "gbps": b / (ms * 1e-3) / 1e9}
         s.close()

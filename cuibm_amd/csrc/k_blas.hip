@@ -122,7 +122,14 @@ __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int *__restrict__ 
     const double stopv = stop ? *stop : 0.0;  // read at once, tested after the first row
     double acc[2] = {0, 0};
     bool tested = false;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    // The first nb blocks (a multiple of 8, so spread evenly over the XCDs) each take a
+    // contiguous chunk of rows, and the chunks of one XCD are contiguous too, so that the
+    // gathers of x from neighbouring rows stay in that XCD's L2; the other blocks write
+    // zero partials.
+    const int nb = min(RED_GRID, (int)((n / BS + 8) / 8 * 8));
+    const int chunk = (n + nb - 1) / nb;
+    const int r0 = (int)blockIdx.x < nb ? xcd_block(nb) * chunk : n, r1 = min(n, r0 + chunk);
+    for (int i = r0 + threadIdx.x; i < r1; i += BS) {
         const double yi = row_dot(rp, ci, v, x, i);
         if (!tested) {
             if (stopv != 0.0) return;  // every thread returns here or below: no barrier is reached

@@ -542,6 +542,7 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
     auto once = [&]() {
         if (k == "spmv_lhs2") launch_spmv<double, double>(LHS2, pressure.p(), t1.p(), s);
         else if (k == "spmv_dot_lhs2") op2->spmv_dot(pressure.p(), t1.p(), rhs2.p(), true, part.p(), s);
+        else if (k == "spmv_dot_lhs1") op1->spmv_dot(uhat.p(), t1.p(), rhs1.p(), true, part.p(), s);
         else if (isCopy) launch_copy(c2.p(), c1.p(), kCopyN, s);
         else if (k == "spmv_lhs1") launch_spmv<double, double>(LHS1, uhat.p(), t1.p(), s);
         else if (k == "rhs1")
@@ -554,6 +555,7 @@ int NavierStokesSolver::timeKernel(const std::string &k, int reps, double *ms, d
     };
     if (k == "spmv_lhs2") b = csr_bytes(LHS2);
     else if (k == "spmv_dot_lhs2") b = op2->spmv_bytes() + 8.0 * numP + 16.0 * RED_GRID;  // + dot operand, partials
+    else if (k == "spmv_dot_lhs1") b = op1->spmv_bytes() + 8.0 * numUV + 16.0 * RED_GRID;
     else if (isCopy) b = 16.0 * kCopyN;
     else if (k == "spmv_lhs1") b = csr_bytes(LHS1);
     else if (k == "rhs1") b = 32.0 * numUV + 8.0 * (2 * nx + 2 * ny) + 4 * 8.0 * (2 * ny + 2 * nx);

@@ -165,7 +165,9 @@ int cuibm_time_steps(cuibm_solver *s, int nsteps, double *ms);
  * return the mean launch time in *ms and the algorithmic bytes per launch in *bytes.
  * kernel: "spmv_lhs2" (the Poisson operator product y = LHS2 x), "spmv_dot_lhs2"
  * (the same product fused with the two BiCGSTAB dot partials, as launched inside the
- * Poisson solve), "spmv_lhs1", "rhs1" (fused intermediate-velocity right-hand side),
+ * Poisson solve), "spmv_lhs1", "spmv_dot_lhs1" (the velocity operator product with the
+ * same two dot partials, as launched inside the velocity solve), "rhs1" (fused
+ * intermediate-velocity right-hand side),
  * "vcycle" (one Poisson preconditioner application), "jacobi0" (one weighted-Jacobi
  * sweep on the finest AMG level), "vc_pre" / "vc_post" (the fused finest-level
  * pre-smoothing + residual and prolongation + post-smoothing kernels of the stencil
