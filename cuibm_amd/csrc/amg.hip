@@ -68,7 +68,9 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
         X s = 0;
         const int e = rp[i + 1];
-        for (int k = rp[i] + lane; k < e; k += LPR) s += (X)v[k] * x[ci[k]];
+        if (LPR == 1) s = csr_row_sum<X>(ci, v, x, rp[i], e);
+        else
+            for (int k = rp[i] + lane; k < e; k += LPR) s += (X)v[k] * x[ci[k]];
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);

@@ -93,10 +93,7 @@ void launch_to_double(double *y, const float *x, long n, hipStream_t s) {
 template <class V, class X>
 __device__ __forceinline__ X row_dot(const int *__restrict__ rp, const int *__restrict__ ci, const V *__restrict__ v,
                                      const X *__restrict__ x, int i) {
-    double s = 0;  // FP64 accumulation also for float operands
-    const int e = rp[i + 1];
-    for (int k = rp[i]; k < e; k++) s += (double)v[k] * (double)x[ci[k]];
-    return (X)s;
+    return (X)csr_row_sum<double>(ci, v, x, rp[i], rp[i + 1]);  // FP64 accumulation also for float operands
 }
 
 template <class V, class X>
