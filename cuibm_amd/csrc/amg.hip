@@ -66,11 +66,7 @@ __global__ void __launch_bounds__(BS) k_csr(int n, const int *__restrict__ rp, c
     const int blk = seg ? (int)blockIdx.x : xcd_block(gridDim.x);
     for (int i = blk * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
-        X s = 0;
-        const int e = rp[i + 1];
-        if (LPR == 1) s = csr_row_sum<X>(ci, v, x, rp[i], e);
-        else
-            for (int k = rp[i] + lane; k < e; k += LPR) s += (X)v[k] * x[ci[k]];
+        X s = csr_lane_sum<X, LPR>(ci, v, rp[i] + lane, rp[i + 1], [&](int c) { return x[c]; });
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
@@ -100,12 +96,8 @@ __global__ void __launch_bounds__(BS) k_rres(int n, int nf, const int *__restric
     const int blk = seg ? (int)blockIdx.x : xcd_block(gridDim.x);
     for (int i = blk * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
-        X s = 0;
-        const int e = rp[i + 1];
-        for (int k = rp[i] + lane; k < e; k += LPR) {
-            const int c = ci[k];
-            s += (X)v[k] * (c < nf ? b[c] : x[c - nf]);
-        }
+        // column c < nf reads b[c], the others x[c - nf]: the address is chosen, not the load
+        X s = csr_lane_sum<X, LPR>(ci, v, rp[i] + lane, rp[i + 1], [&](int c) { return *(c < nf ? b + c : x + (c - nf)); });
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);
@@ -149,12 +141,7 @@ __global__ void __launch_bounds__(BS) k_pre2(int n, const int *__restrict__ rp, 
     const int blk = seg ? (int)blockIdx.x : xcd_block(gridDim.x);
     for (int i = blk * per_block + (int)threadIdx.x / LPR; i < n; i += gridDim.x * per_block) {
         if (seg && *cstop[seg[i / BATCH_PAD]] != 0.0) continue;  // a stopped case of a batch (whole row group)
-        X s = 0;
-        const int e = rp[i + 1];
-        for (int k = rp[i] + lane; k < e; k += LPR) {
-            const int j = ci[k];
-            s += (X)v[k] * (omega * (X)dinv[j] * b[j]);
-        }
+        X s = csr_lane_sum<X, LPR>(ci, v, rp[i] + lane, rp[i + 1], [&](int j) { return omega * (X)dinv[j] * b[j]; });
         if (LPR > 1)
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LPR);

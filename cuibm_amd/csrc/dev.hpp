@@ -50,35 +50,44 @@ inline unsigned xcd_blocks(long blocks) {
     return (unsigned)((b + 7) / 8 * 8);
 }
 
-// s = sum over k in [b, e) of v[k] x[ci[k]], added in k order in type S. The index and
-// value loads of a group of 5 entries (a 5-point row) are issued together, then their x
-// loads, before the first add: a loop that waits for each entry's loads in turn keeps one
-// row about two memory latencies per entry long, and with every block of a launch
-// resident the cache lines of a row's first entries can leave L1 and L2 before its last
-// entries read them. The loads are unconditional (a short group re-reads its last entry)
-// so that no branch splits them; only the adds are masked.
-template <class S, class V, class X>
-__device__ __forceinline__ S csr_row_sum(const int *__restrict__ ci, const V *__restrict__ v,
-                                         const X *__restrict__ x, int b, int e) {
-    constexpr int G = 5;
+// s = sum over k = b, b + STRIDE, ... < e of v[k] g(ci[k]), added in k order in type S
+// (STRIDE > 1: one lane's share of a row that several lanes split). The index and value
+// loads of a group of entries (5, a 5-point row, for one lane per row; 4 for a lane of a
+// split row) are issued together, then the gathers g(c), before the first add: a loop
+// that waits for each entry's loads in turn keeps a row about two memory latencies per
+// entry long, and with every block of a launch resident the cache lines of a row's first
+// entries can leave L1 and L2 before its last entries read them. The loads are
+// unconditional (a short group re-reads its last entry, and g must read only what its
+// column allows) so that no branch splits them; only the adds are masked.
+template <class S, int STRIDE, class V, class Gather>
+__device__ __forceinline__ S csr_lane_sum(const int *__restrict__ ci, const V *__restrict__ v, int b, int e,
+                                          Gather g) {
+    constexpr int G = STRIDE == 1 ? 5 : 4;
     S s = 0;
-    for (int k0 = b; k0 < e; k0 += G) {
-        const int last = min(G, e - k0) - 1;
+    for (int k0 = b; k0 < e; k0 += G * STRIDE) {
+        const int last = min(G, (e - k0 + STRIDE - 1) / STRIDE) - 1;
         int c[G];
         S a[G], xv[G];
 #pragma unroll
         for (int j = 0; j < G; j++) {
-            const int k = k0 + min(j, last);
+            const int k = k0 + min(j, last) * STRIDE;
             c[j] = ci[k];
             a[j] = (S)v[k];
         }
 #pragma unroll
-        for (int j = 0; j < G; j++) xv[j] = (S)x[c[j]];
+        for (int j = 0; j < G; j++) xv[j] = g(c[j]);
 #pragma unroll
         for (int j = 0; j < G; j++)
             if (j <= last) s += a[j] * xv[j];
     }
     return s;
+}
+
+// s = sum over k in [b, e) of v[k] x[ci[k]], added in k order in type S
+template <class S, class V, class X>
+__device__ __forceinline__ S csr_row_sum(const int *__restrict__ ci, const V *__restrict__ v,
+                                         const X *__restrict__ x, int b, int e) {
+    return csr_lane_sum<S, 1>(ci, v, b, e, [&](int c) { return (S)x[c]; });
 }
 
 template <class T>
